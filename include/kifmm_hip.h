@@ -1,0 +1,112 @@
+/*
+ * kifmm_hip.h -- GPU evaluator of the MI355X KIFMM path (C ABI).
+ *
+ * The reference plugs an evaluator in at link time through
+ *   FMMWrapper_t* preproc (AllNodes*)   fmm/evaluate.h:18, body fmm/evaluate--cuda.cu:1441-2239
+ *   int run (FMMWrapper_t*)             fmm/evaluate.h:23, body fmm/run--cuda.cu:27-345
+ * kifmm_gpu_create replaces preproc (flatten + upload, library-owned device
+ * memory); kifmm_gpu_evaluate replaces run (up, U, V, W, X, down, copy;
+ * phase timing as fmm/run--cuda.cu:323-340 with the copy actually timed);
+ * kifmm_gpu_destroy frees what create allocated (the reference never frees).
+ *
+ * All entry points return 0 on success or a negative KIFMM_E* code, never
+ * exit.  A handle is not thread-safe; several handles may coexist.  Each
+ * handle owns one HIP stream at default priority on its device.
+ */
+#ifndef KIFMM_HIP_H
+#define KIFMM_HIP_H
+
+#include "kifmm_host.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct kifmm_gpu kifmm_gpu;
+
+enum {
+  KIFMM_PREC_FP64 = 0, /* plain FP64 path */
+  KIFMM_PREC_MIXED = 1 /* NP <= 4: FP32 seed + FP64 Newton P2P, FP32 complex M2L */
+};
+
+typedef struct kifmm_gpu_options {
+  int device;    /* HIP device ordinal */
+  int precision; /* KIFMM_PREC_* */
+  int reserved[6];
+} kifmm_gpu_options;
+
+/* Milliseconds, measured with HIP events on the handle's stream. */
+typedef struct kifmm_phase_times {
+  double up;        /* S2M + M2M                          */
+  double ulist;     /* U-list P2P                         */
+  double vlist;     /* sum of the four V sub-phases       */
+  double v_fft_src; /* surface -> grid scatter + forward DFT */
+  double v_trans;   /* translation spectra (316 vectors)  */
+  double v_hadamard;/* spectral multiply-accumulate       */
+  double v_ifft;    /* inverse DFT + check-surface gather */
+  double wlist;
+  double xlist;
+  double down;      /* L2L + DC2DE + L2T                  */
+  double copy;      /* device -> host potentials          */
+  double total;     /* first event to last event          */
+  double p2p_kernel;/* the U-list kernel alone            */
+  double reserved[3];
+} kifmm_phase_times;
+
+/* Algorithmic work counts of the uploaded problem (for roofline figures). */
+typedef struct kifmm_work_counts {
+  double u_pairs;      /* sum over U entries of trg_num * src_num  */
+  double s2m_pairs;    /* n_uc * src_num over source leaves        */
+  double l2t_pairs;    /* n_e * trg_num over target leaves         */
+  double w_pairs, x_pairs;
+  double v_entries;    /* V-list nnz                                */
+  double v_cmac;       /* complex multiply-adds in the Hadamard kernel as launched */
+  double v_cmac_ref;   /* (eff/2) * V nnz, the reference count      */
+  double gemm_flops;   /* 2*m*k per applied operator column         */
+  double fft_bytes;    /* HBM bytes of the forward + inverse DFT kernels */
+  double hadamard_bytes; /* compulsory HBM bytes of the Hadamard kernel */
+  double reserved[5];
+} kifmm_work_counts;
+
+int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops,
+                     const kifmm_gpu_options* opts, kifmm_gpu** out);
+/* Evaluate the potentials.  pot_leaf (host, n_trg doubles, gathered target
+ * order) may be NULL: the result then stays in device memory and the copy
+ * phase is skipped.  times may be NULL. */
+int kifmm_gpu_evaluate(kifmm_gpu* g, double* pot_leaf, kifmm_phase_times* times);
+int kifmm_gpu_destroy(kifmm_gpu* g);
+
+/* Phase selection, the counterpart of the reference's separately callable
+ * phase functions up_calc__gpu, ulist_calc__gpu, vlist_calc__gpu,
+ * wlist_calc__gpu, xlist_calc__gpu, down_calc__gpu (fmm/evaluate.h:42-57).
+ * Accumulators are zeroed first; a phase whose inputs were not produced in
+ * the same call sees zeros. */
+enum {
+  KIFMM_PHASE_UP = 1,
+  KIFMM_PHASE_U = 2,
+  KIFMM_PHASE_V = 4,
+  KIFMM_PHASE_W = 8,
+  KIFMM_PHASE_X = 16,
+  KIFMM_PHASE_DOWN = 32,
+  KIFMM_PHASE_ALL = 63
+};
+int kifmm_gpu_evaluate_phases(kifmm_gpu* g, unsigned phases, double* pot_leaf, kifmm_phase_times* times);
+
+int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out);
+/* Derives and bounds-checks the device work lists of a tree view on the
+ * host only (no device call) -- the checks kifmm_gpu_create runs before
+ * it uploads anything.  out may be NULL. */
+int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifmm_work_counts* out);
+/* Test access to device intermediates after an evaluate, node-major with
+ * n_e doubles per node: which = 0 upward equivalent densities,
+ * 1 downward check values, 2 downward equivalent densities, 3 potentials
+ * (n_trg doubles, gathered order). */
+int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long count);
+
+/* 1 if a HIP device is usable from this process. */
+int kifmm_gpu_available(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

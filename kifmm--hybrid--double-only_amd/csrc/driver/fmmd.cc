@@ -1,0 +1,94 @@
+/*
+ * fmmd.cc -- the fmmd--hip command line, a drop-in for fmmd--cuda.
+ *
+ * Same surface as the reference driver (fmm/driver.cc:211-373):
+ *   fmmd--hip <#pts> {uniform,ellipseUniformAngles,ellipsoid,gaussmix} <pts-per-box>
+ * environment NP (3, 4, 6), NV (validation samples), C (sources =
+ * targets), SEED (default 26, the reference's srand48(26)).
+ * Steps: generate + trees + lists + gather, operators, kifmm_gpu_create
+ * (preproc), kifmm_gpu_evaluate (run, with the copy back timed),
+ * scatter to input order, validate, print the reference's estimates.
+ */
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <vector>
+
+#include "kifmm_hip.h"
+
+static void usage(const char* prog) { fprintf(stderr, "usage: %s <N> <distribution> <pts/box> \n", prog); }
+
+int main(int argc, char** argv) {
+  if (argc != 4) {
+    usage(argv[0]);
+    return -1;
+  }
+  const int n = atoi(argv[1]);
+  const char* dist = argv[2];
+  const int q = atoi(argv[3]);
+  kifmm_problem_opts po = {-1, -1, -1, 0};
+  fprintf(stderr, "Building trees and interaction lists...\n");
+  kifmm_problem* p = kifmm_problem_create(n, dist, q, &po);
+  if (!p) {
+    fprintf(stderr, "error: %s\n", kifmm_last_error());
+    return 1;
+  }
+  kifmm_tree_view tv;
+  kifmm_operators ops;
+  fprintf(stderr, "Setup for translations....\n");
+  if (kifmm_problem_view(p, &tv) || kifmm_problem_operators(p, &ops)) {
+    fprintf(stderr, "error: %s\n", kifmm_last_error());
+    return 1;
+  }
+  fprintf(stderr, "Preprocessing...\n");
+  kifmm_gpu_options go;
+  memset(&go, 0, sizeof go);
+  kifmm_gpu* g = nullptr;
+  if (kifmm_gpu_create(&tv, &ops, &go, &g) != KIFMM_OK) {
+    fprintf(stderr, "error: %s\n", kifmm_last_error());
+    return 1;
+  }
+  fprintf(stderr, "Finished setup.\n");
+  fprintf(stderr, "Evaluation...\n");
+  std::vector<double> pot_leaf(n), pot(n);
+  kifmm_phase_times t;
+  if (kifmm_gpu_evaluate(g, pot_leaf.data(), &t) != KIFMM_OK) {
+    fprintf(stderr, "error: %s\n", kifmm_last_error());
+    return 1;
+  }
+  kifmm_problem_scatter(p, pot_leaf.data(), pot.data());
+  const double tot = (t.up + t.ulist + t.vlist + t.wlist + t.xlist + t.down + t.copy) * 1e-3;
+  fprintf(stderr, "=== Statistics ===\n");
+  fprintf(stdout, "Code: hip-gfx950\n");
+  fprintf(stdout, "Floating-point word size: %lu bytes\n", sizeof(double));
+  fprintf(stderr, "== Timing for FMM3d ==\n");
+  const double ph[7] = {t.up, t.ulist, t.vlist, t.wlist, t.xlist, t.down, t.copy};
+  const char* nm[7] = {"Up    ", "U list", "V list", "W list", "X list", "Down  ", "Copy  "};
+  for (int i = 0; i < 7; i++) fprintf(stderr, "  %s: %g secs (%.1f%%)\n", nm[i], ph[i] * 1e-3, ph[i] * 1e-3 / tot * 100);
+  fprintf(stderr, "  ==> Total Execution Time: %g secs\n", tot);
+  fprintf(stderr, "  ==> Throughput: %.2f Mpoints/s\n", n / tot * 1e-6);
+  const int nv = kifmm_getenv_validate();
+  const double rerr = kifmm_problem_validate(p, pot.data(), nv, nullptr, nullptr);
+  kifmm_work_counts wc;
+  kifmm_gpu_work_counts(g, &wc);
+  const int eff = 2 * (2 * tv.np) * (2 * tv.np) * (tv.np + 1);
+  double bytes_u = 0, bytes_v = 0;
+  for (int i = 0; i < tv.n_nodes; i++) {
+    bytes_u += 4.0 * 8 * tv.trg_num[i];
+    for (int k = tv.u_ptr[i]; k < tv.u_ptr[i + 1]; k++) bytes_u += 4.0 * 8 * tv.src_num[tv.u_idx[k]];
+    bytes_v += 8.0 * eff + 2.0 * 8 * eff * (tv.v_ptr[i + 1] - tv.v_ptr[i]);
+  }
+  const double flops_u = 11.0 * wc.u_pairs, flops_v = 4.0 * eff * wc.v_entries;
+  fprintf(stdout, "Inputs: %d | %s | %d \n", n, dist, q);
+  fprintf(stdout, "Estimated no. of flops (Ulist): %g\n", flops_u);
+  fprintf(stdout, "Estimated no. of flops (Vlist): %g\n", flops_v);
+  fprintf(stdout, "Lower-bound on maximum memory traffic (Ulist): %g GB\n", 1e-9 * bytes_u);
+  fprintf(stdout, "Lower-bound on maximum memory traffic (Vlist): %g GB\n", 1e-9 * bytes_v);
+  fprintf(stdout, "Lower-bound on average intensity (Ulist): %.2f flops / byte\n", flops_u / bytes_u);
+  fprintf(stdout, "Lower-bound on average intensity (Vlist): %.2f flops / byte\n", flops_v / bytes_v);
+  fprintf(stderr, "Relative error: %e\n", rerr);
+  kifmm_gpu_destroy(g);
+  kifmm_problem_destroy(p);
+  return 0;
+}

@@ -1,0 +1,678 @@
+/*
+ * kernels.hpp -- gfx950 kernels of the KIFMM evaluation path (FP64 plain path).
+ *
+ * Stage map (reference kernel it replaces, fmm/cuda.cu):
+ *   k_p2p_ulist ....... ulist_eval__gpu             (604-663)   U-list P2P
+ *   k_s2m ............. up_eval__gpu, first half    (108-190)   sources -> UC check potential
+ *   k_gemm_cols ....... the dgemv loops of up_eval__gpu, up_eval__gpu_non_leaf,
+ *                       down_eval__gpu_depth_2 / _non_leaves    (430-528, 1682-1801)
+ *                       as FP64 MFMA GEMMs over a column list
+ *   k_dft_fwd ......... compute_fft_src__gpu_eval + cuFFT D2Z   (718-871)
+ *   k_trans ........... compute_fft_trans__gpu_eval + D2Z x343  (874-963)
+ *   k_hadamard ........ vlist_calc__gpu_eval                    (1200-1240)
+ *   k_dft_inv ......... scale_nrmfc + cuFFT Z2D + regVal2SamVal (965-1196)
+ *   k_wlist ........... wlist_eval__gpu                         (2070-2197)
+ *   k_xlist ........... xlist_eval__gpu                         (2380-2493)
+ *   k_l2t ............. down_eval__gpu_leaves                   (1804-1872)
+ *
+ * Point evaluations share one scheme: a wave owns 8 targets held in
+ * scalar registers, its 64 lanes stride over a source tile staged in LDS,
+ * and the 8 per-lane partial sums are combined by a 6-step butterfly
+ * (recursive halving) across the wave.  Sources are (x, y, z, w) double4
+ * records; tiles are padded with zero-weight points far outside the root
+ * box so no per-pair bounds checks are needed.
+ */
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace kifmm_dev {
+
+constexpr double OOFP_R = 0.079577471545947667884441881686257181; /* 1/(4 pi) */
+constexpr double FAR = 1.0e8; /* coordinate of padding sources */
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+/* x^(-1/2): hardware estimate refined by one third-order step of the
+ * Newton-Raphson family for x^(-1/2):  e = 1 - x y^2,
+ * y <- y (1 + e/2 + 3e^2/8), which cubes the relative error. */
+__device__ __forceinline__ double rsqrt_fp64(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  double h = x * y;
+  double e = __builtin_fma(-h, y, 1.0);
+  double p = __builtin_fma(e, 0.375, 0.5);
+  return __builtin_fma(y * e, p, y);
+}
+
+template <bool MASK>
+__device__ __forceinline__ double inv_r(double r2) {
+  double y = rsqrt_fp64(r2);
+  if (MASK) y = (r2 > 0.0) ? y : 0.0;
+  return y;
+}
+
+/* 8 partial sums per lane -> lane l holds the wave total of target (l >> 3)
+ * in lanes with (l & 7) == 0.  Fixed order, deterministic. */
+__device__ __forceinline__ double wave_reduce8(const double acc[8], int lane) {
+  double a[4], b[2], c;
+#pragma unroll
+  for (int i = 0; i < 4; i++) {
+    const bool hi = lane & 32;
+    double send = hi ? acc[i] : acc[i + 4];
+    double keep = hi ? acc[i + 4] : acc[i];
+    a[i] = keep + __shfl_xor(send, 32);
+  }
+#pragma unroll
+  for (int i = 0; i < 2; i++) {
+    const bool hi = lane & 16;
+    double send = hi ? a[i] : a[i + 2];
+    double keep = hi ? a[i + 2] : a[i];
+    b[i] = keep + __shfl_xor(send, 16);
+  }
+  {
+    const bool hi = lane & 8;
+    double send = hi ? b[0] : b[1];
+    double keep = hi ? b[1] : b[0];
+    c = keep + __shfl_xor(send, 8);
+  }
+  c += __shfl_xor(c, 4);
+  c += __shfl_xor(c, 2);
+  c += __shfl_xor(c, 1);
+  return c;
+}
+
+/* acc[t] += sum over LDS sources [0, ns) (ns multiple of 64) of w / |x_t - s| */
+template <bool MASK>
+__device__ __forceinline__ void eval8_lds(const double4* __restrict__ src, int ns, const double tx[8],
+                                          const double ty[8], const double tz[8], double acc[8],
+                                          int lane) {
+  for (int s = lane; s < ns; s += 64) {
+    const double4 p = src[s];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const double dx = tx[t] - p.x, dy = ty[t] - p.y, dz = tz[t] - p.z;
+      const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      acc[t] = __builtin_fma(p.w, inv_r<MASK>(r2), acc[t]);
+    }
+  }
+}
+
+/* Loads n points of a list into the LDS tile (n <= cap, padded to a
+ * multiple of 64 with zero-weight far points).  Either gathered sources
+ * (surf == nullptr) or a surface c + r * SP with densities den. */
+__device__ __forceinline__ int stage_sources(double4* tile, int cap, const double4* __restrict__ src4, int beg,
+                                             int n, const double* __restrict__ surf, int ld_surf, double4 g,
+                                             const double* __restrict__ den, int tid, int nthreads) {
+  const int npad = min(cap, ((n + 63) / 64) * 64);
+  for (int k = tid; k < npad; k += nthreads) {
+    double4 v = make_double4(FAR, FAR, FAR, 0.0);
+    if (k < n) {
+      if (surf) v = make_double4(g.x + g.w * surf[k], g.y + g.w * surf[ld_surf + k], g.z + g.w * surf[2 * ld_surf + k], den[k]);
+      else v = src4[beg + k];
+    }
+    tile[k] = v;
+  }
+  return npad;
+}
+
+/* ------------------------------------------------------------------ U-list */
+
+/* One block = 4 waves = up to 32 targets of one target leaf.  item = {leaf
+ * range index, first target, target count}.  The U-list of the leaf is a
+ * list of merged source ranges {beg, num}. */
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items,
+                                                   const int* __restrict__ rng_ptr,
+                                                   const int2* __restrict__ rng,
+                                                   const double4* __restrict__ src4,
+                                                   const double4* __restrict__ trg4,
+                                                   double* __restrict__ pot) {
+  __shared__ double4 tile[256];
+  __shared__ int pre[65];
+  __shared__ int rbeg[64];
+  const int4 it = items[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int rfirst = rng_ptr[it.x], rlast = rng_ptr[it.x + 1];
+  /* the wave's 8 targets; slots past the item's count re-read the item's
+   * last target (always inside [it.y, it.y + it.z)) and are not stored */
+  const int t0 = it.y + 8 * wave;
+  const int nt = min(8, it.z - 8 * wave);
+  double tx[8], ty[8], tz[8], acc[8];
+#pragma unroll
+  for (int t = 0; t < 8; t++) {
+    const double4 q = trg4[it.y + min(8 * wave + t, it.z - 1)];
+    tx[t] = q.x; ty[t] = q.y; tz[t] = q.z; acc[t] = 0.0;
+  }
+  for (int r0 = rfirst; r0 < rlast; r0 += 64) {
+    const int nr = min(64, rlast - r0);
+    __syncthreads();
+    if (tid == 0) {
+      int s = 0;
+      for (int r = 0; r < nr; r++) {
+        const int2 g = rng[r0 + r];
+        pre[r] = s;
+        rbeg[r] = g.x;
+        s += g.y;
+      }
+      pre[nr] = s;
+    }
+    __syncthreads();
+    const int total = pre[nr];
+    for (int base = 0; base < total; base += 256) {
+      const int p = base + tid;
+      double4 v = make_double4(FAR, FAR, FAR, 0.0);
+      if (p < total) {
+        int lo = 0, hi = nr - 1; /* last range with pre[r] <= p */
+        while (lo < hi) {
+          const int mid = (lo + hi + 1) >> 1;
+          if (pre[mid] <= p) lo = mid; else hi = mid - 1;
+        }
+        v = src4[rbeg[lo] + (p - pre[lo])];
+      }
+      tile[tid] = v;
+      __syncthreads();
+      const int ns = min(256, ((total - base + 63) / 64) * 64);
+      eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
+      __syncthreads();
+    }
+  }
+  const double sum = wave_reduce8(acc, lane);
+  const int t = lane >> 3;
+  if ((lane & 7) == 0 && t < nt) pot[t0 + t] = OOFP_R * sum;
+}
+
+/* ------------------------------------------------------------------ S2M */
+
+/* One block per source leaf: UC check potential at c + r * SP_UC from the
+ * leaf's sources (up_eval__gpu, fmm/cuda.cu:108-190 first half). */
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
+                                             const double4* __restrict__ geo, /* node centre xyz + radius */
+                                             const double* __restrict__ surf_uc, /* 3 x ld, unit surface */
+                                             int n_uc, int ld_uc_surf,
+                                             const double4* __restrict__ src4,
+                                             double* __restrict__ chk, int ld_chk) {
+  __shared__ double4 tile[512];
+  const int4 it = items[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const double4 g = geo[it.x];
+  double* out = chk + (size_t)it.w * ld_chk;
+  for (int base = 0; base < max(it.z, 1); base += 512) {
+    __syncthreads();
+    const int ns = stage_sources(tile, 512, src4, it.y + base, it.z - base, nullptr, 0, g, nullptr, tid, 256);
+    __syncthreads();
+    const int ngroups = (n_uc + 7) / 8;
+    for (int grp = wave; grp < ngroups; grp += 4) {
+      double tx[8], ty[8], tz[8], acc[8];
+#pragma unroll
+      for (int t = 0; t < 8; t++) {
+        const int j = min(grp * 8 + t, n_uc - 1);
+        tx[t] = g.x + g.w * surf_uc[j];
+        ty[t] = g.y + g.w * surf_uc[ld_uc_surf + j];
+        tz[t] = g.z + g.w * surf_uc[2 * ld_uc_surf + j];
+        acc[t] = 0.0;
+      }
+      eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
+      const double sum = wave_reduce8(acc, lane);
+      const int j = grp * 8 + (lane >> 3);
+      if ((lane & 7) == 0 && j < n_uc) out[j] = (base == 0 ? 0.0 : out[j]) + OOFP_R * sum;
+    }
+  }
+}
+
+/* ------------------------------------------------------------ GEMM (MFMA) */
+
+/* out[:, j] = (acc ? out[:, j] : 0) + 2^sexp[j] * A[M x K] * in[:, j]
+ * A column-major with lda rows (zero-padded to a multiple of 16) and K a
+ * multiple of 4 (zero-padded columns).  Column j reads K doubles at
+ * in + in_off[j] and writes M doubles at out + out_off[j].  One block of
+ * four waves owns 16 columns and 4*RT row tiles (blockIdx.y picks the
+ * group); wave w owns tiles w, w+4, ... of its group.
+ * v_mfma_f64_16x16x4f64 operand layout (measured on gfx950 by
+ * test_mfma_f64_operand_layout): A[i][k] in lane i + 16k, B[k][j] in lane
+ * j + 16k, D[i][j] in lane j + 16(i%4), accumulator element i/4. */
+constexpr int GEMM_KC = 64;
+template <int RT>
+__global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* __restrict__ A, int lda,
+                                                   int ncols, const int* __restrict__ in_off,
+                                                   const int* __restrict__ out_off,
+                                                   const int* __restrict__ sexp,
+                                                   const double* __restrict__ in,
+                                                   double* __restrict__ out, int accumulate) {
+  __shared__ double Bs[GEMM_KC][17];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int col0 = blockIdx.x * 16;
+  const int nrt = (M + 15) / 16;
+  const int rt0 = blockIdx.y * 4 * RT;
+  v4d acc[RT];
+#pragma unroll
+  for (int r = 0; r < RT; r++) acc[r] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < K; k0 += GEMM_KC) {
+    for (int e = tid; e < GEMM_KC * 16; e += 256) {
+      const int c = e / GEMM_KC, k = e % GEMM_KC, col = col0 + c;
+      double v = 0.0;
+      if (col < ncols && k0 + k < K) v = in[(size_t)in_off[col] + k0 + k];
+      Bs[k][c] = v;
+    }
+    __syncthreads();
+    const int kend = min(GEMM_KC, K - k0);
+    for (int kk = 0; kk < kend; kk += 4) {
+      const double b = Bs[kk + (lane >> 4)][lane & 15];
+#pragma unroll
+      for (int r = 0; r < RT; r++) {
+        const int rt = rt0 + wave + 4 * r;
+        if (rt < nrt) {
+          const double a = A[(size_t)(rt * 16 + (lane & 15)) + (size_t)(k0 + kk + (lane >> 4)) * lda];
+          acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  const int col = col0 + (lane & 15);
+  if (col >= ncols) return;
+  const double s = ldexp(1.0, sexp[col]);
+  double* o = out + out_off[col];
+#pragma unroll
+  for (int r = 0; r < RT; r++) {
+    const int rt = rt0 + wave + 4 * r;
+    if (rt >= nrt) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = rt * 16 + (lane >> 4) + 4 * i;
+      if (row < M) {
+        const double v = s * acc[r][i];
+        o[row] = accumulate ? o[row] + v : v;
+      }
+    }
+  }
+}
+
+/* ---------------------------------------------------------- V-list DFTs */
+
+/* Grid geometry for accuracy NP: N = 2 NP points per axis, H = N/2 + 1
+ * half-spectrum length along x, F = N*N*H complex frequencies, the surface
+ * sits at grid offset NP/2 (fmm/trans.cc:395-417). */
+template <int NP>
+struct Grid {
+  static constexpr int N = 2 * NP, H = NP + 1, F = N * N * H, OFF = NP / 2;
+  static constexpr int NE = NP * NP * NP - (NP - 2) * (NP - 2) * (NP - 2);
+};
+
+/* Forward transform of the 8 sibling equivalent densities of one child
+ * block: scatter 2^depth * uden onto the (2NP)^3 grid (only the NP^3 cube
+ * at offset NP/2 is non-zero, so the three axis passes are pruned to it)
+ * and produce the r2c spectrum [kz][ky][kx<=N/2] (FFTW layout), stored
+ * interleaved by octant: X[blk][f][oct]. */
+template <int NP>
+__global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks, /* first child node per block */
+                                                 const int* __restrict__ xslot,  /* X slot per block */
+                                                 const int* __restrict__ depth,
+                                                 const double* __restrict__ uden, int ld_e,
+                                                 const double2* __restrict__ tw, /* W^m, m < N */
+                                                 const int* __restrict__ cube2surf, /* NP^3: sample id or -1 */
+                                                 double2* __restrict__ X) {
+  using G = Grid<NP>;
+  constexpr int N = G::N, H = G::H, F = G::F, NE = G::NE, OFF = G::OFF;
+  __shared__ double gs[8][NE];
+  __shared__ double2 A[8][NP][NP][H];  /* [oct][iz][iy][kx] */
+  __shared__ double2 B[8][NP][N][H];   /* [oct][iz][ky][kx] */
+  __shared__ double2 w[N];
+  __shared__ int c2s[NP * NP * NP];
+  const int tid = threadIdx.x;
+  const int first = blocks[blockIdx.x];
+  if (tid < N) w[tid] = tw[tid];
+  for (int e = tid; e < NP * NP * NP; e += blockDim.x) c2s[e] = cube2surf[e];
+  for (int e = tid; e < 8 * NE; e += blockDim.x) {
+    const int o = e / NE, j = e % NE, node = first + o;
+    gs[o][j] = ldexp(uden[(size_t)node * ld_e + j], depth[node]);
+  }
+  __syncthreads();
+  /* x pass: real -> half spectrum, input support x in [OFF, OFF+NP) */
+  for (int e = tid; e < 8 * NP * NP * H; e += blockDim.x) {
+    const int kx = e % H, iy = (e / H) % NP, iz = (e / (H * NP)) % NP, o = e / (H * NP * NP);
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int ix = 0; ix < NP; ix++) {
+      const int s = c2s[(ix * NP + iy) * NP + iz];
+      const double v = (s >= 0) ? gs[o][s] : 0.0;
+      const double2 t = w[(kx * (ix + OFF)) % N];
+      re = __builtin_fma(v, t.x, re);
+      im = __builtin_fma(v, t.y, im);
+    }
+    A[o][iz][iy][kx] = make_double2(re, im);
+  }
+  __syncthreads();
+  /* y pass */
+  for (int e = tid; e < 8 * NP * N * H; e += blockDim.x) {
+    const int kx = e % H, ky = (e / H) % N, iz = (e / (H * N)) % NP, o = e / (H * N * NP);
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int iy = 0; iy < NP; iy++) {
+      const double2 a = A[o][iz][iy][kx];
+      const double2 t = w[(ky * (iy + OFF)) % N];
+      re = __builtin_fma(a.x, t.x, __builtin_fma(-a.y, t.y, re));
+      im = __builtin_fma(a.x, t.y, __builtin_fma(a.y, t.x, im));
+    }
+    B[o][iz][ky][kx] = make_double2(re, im);
+  }
+  __syncthreads();
+  /* z pass, written coalesced as X[slot][f][oct] */
+  double2* out = X + (size_t)xslot[blockIdx.x] * F * 8;
+  for (int e = tid; e < 8 * F; e += blockDim.x) {
+    const int o = e & 7, f = e >> 3;
+    const int kx = f % H, ky = (f / H) % N, kz = f / (H * N);
+    double re = 0.0, im = 0.0;
+#pragma unroll
+    for (int iz = 0; iz < NP; iz++) {
+      const double2 b = B[o][iz][ky][kx];
+      const double2 t = w[(kz * (iz + OFF)) % N];
+      re = __builtin_fma(b.x, t.x, __builtin_fma(-b.y, t.y, re));
+      im = __builtin_fma(b.x, t.y, __builtin_fma(b.y, t.x, im));
+    }
+    out[e] = make_double2(re, im);
+  }
+}
+
+/* Translation spectra (compute_fft_trans__gpu_eval + D2Z, fmm/cuda.cu:874-963):
+ * for id = (i1+3) + 7(i2+3) + 49(i3+3) with max|i| > 1,
+ * tt[g] = 1/(4 pi |2 i - RP_g|) on the wrapped (2NP)^3 grid, full r2c.
+ * Output layout T[f][id] so that a wave reads one frequency's 343 entries
+ * as uniform (scalar) loads. */
+template <int NP>
+__global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, double2* __restrict__ T) {
+  using G = Grid<NP>;
+  constexpr int N = G::N, H = G::H, F = G::F;
+  __shared__ double g[N * N * N];
+  __shared__ double2 a[N * N * H];
+  __shared__ double2 b[N * N * H];
+  __shared__ double2 w[N];
+  const int id = blockIdx.x, tid = threadIdx.x;
+  const int i1 = id % 7 - 3, i2 = (id / 7) % 7 - 3, i3 = id / 49 - 3;
+  const bool valid = abs(i1) > 1 || abs(i2) > 1 || abs(i3) > 1;
+  if (!valid) {
+    for (int f = tid; f < F; f += blockDim.x) T[(size_t)f * 343 + id] = make_double2(0.0, 0.0);
+    return;
+  }
+  if (tid < N) w[tid] = tw[tid];
+  const double step = 2.0 / (NP - 1);
+  for (int e = tid; e < N * N * N; e += blockDim.x) {
+    const int i = e % N, j = (e / N) % N, k = e / (N * N);
+    const int gi = i < NP ? i : i - N, gj = j < NP ? j : j - N, gk = k < NP ? k : k - N;
+    const double dx = i1 * 2.0 - gi * step, dy = i2 * 2.0 - gj * step, dz = i3 * 2.0 - gk * step;
+    g[e] = OOFP_R / sqrt(dx * dx + dy * dy + dz * dz);
+  }
+  __syncthreads();
+  for (int e = tid; e < N * N * H; e += blockDim.x) {
+    const int kx = e % H, yz = e / H;
+    double re = 0.0, im = 0.0;
+    for (int x = 0; x < N; x++) {
+      const double v = g[yz * N + x];
+      const double2 t = w[(kx * x) % N];
+      re = __builtin_fma(v, t.x, re);
+      im = __builtin_fma(v, t.y, im);
+    }
+    a[e] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int e = tid; e < N * N * H; e += blockDim.x) {
+    const int kx = e % H, ky = (e / H) % N, z = e / (H * N);
+    double re = 0.0, im = 0.0;
+    for (int y = 0; y < N; y++) {
+      const double2 v = a[(z * N + y) * H + kx];
+      const double2 t = w[(ky * y) % N];
+      re = __builtin_fma(v.x, t.x, __builtin_fma(-v.y, t.y, re));
+      im = __builtin_fma(v.x, t.y, __builtin_fma(v.y, t.x, im));
+    }
+    b[e] = make_double2(re, im);
+  }
+  __syncthreads();
+  for (int e = tid; e < N * N * H; e += blockDim.x) {
+    const int kx = e % H, ky = (e / H) % N, kz = e / (H * N);
+    double re = 0.0, im = 0.0;
+    for (int z = 0; z < N; z++) {
+      const double2 v = b[(z * N + ky) * H + kx];
+      const double2 t = w[(kz * z) % N];
+      re = __builtin_fma(v.x, t.x, __builtin_fma(-v.y, t.y, re));
+      im = __builtin_fma(v.x, t.y, __builtin_fma(v.y, t.x, im));
+    }
+    T[(size_t)e * 343 + id] = make_double2(re, im);
+  }
+}
+
+/* Spectral M2L for all 8 children of a parent at once.  Lane = parent,
+ * block row = one frequency f (uniform, so translation entries are scalar
+ * loads).  The V-list of child s of parent p is every child c of p's 26
+ * same-depth neighbours o that is not adjacent to s (fmm/input.cc:314-336).
+ * With child offset d = 2o + (c - s) in [-3,3]^3 the translation id is
+ * (dx+3) + 7(dy+3) + 49(dz+3) (fmm/evaluate--cuda.cu:1713-1723), i.e. a
+ * per-neighbour base plus a per-(c,s) constant.  Pairs that are adjacent
+ * (max|d| < 2) hit the 27 near-field ids, whose spectra are exactly zero,
+ * so every (c,s) pair is computed unconditionally and adds exactly 0 where
+ * the reference has no V entry.  Missing neighbours point at the all-zero
+ * X slot 0. */
+template <int NP>
+__global__ __launch_bounds__(64) void k_hadamard(int nparents, const int* __restrict__ nbr, /* 27 X slots per parent */
+                                                 const int* __restrict__ yslot,            /* Y slot per parent */
+                                                 const double2* __restrict__ T,
+                                                 const double2* __restrict__ X,
+                                                 double2* __restrict__ Y) {
+  using G = Grid<NP>;
+  constexpr int F = G::F;
+  const int p = blockIdx.x * 64 + threadIdx.x;
+  const int f = blockIdx.y;
+  const double2* __restrict__ Tf = T + (size_t)f * 343;
+  const int pp = min(p, nparents - 1);
+  double yr[8], yi[8];
+#pragma unroll
+  for (int s = 0; s < 8; s++) { yr[s] = 0.0; yi[s] = 0.0; }
+#pragma unroll 1
+  for (int o = 0; o < 27; o++) {
+    if (o == 13) continue; /* the parent itself: its children are all adjacent */
+    const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
+    const double2* __restrict__ To = Tf + (2 * ox + 3) + 7 * (2 * oy + 3) + 49 * (2 * oz + 3);
+    const int slot = nbr[(size_t)pp * 27 + o];
+    const double2* xb = X + ((size_t)slot * F + f) * 8;
+    double xr[8], xi[8];
+#pragma unroll
+    for (int c = 0; c < 8; c++) { const double2 v = xb[c]; xr[c] = v.x; xi[c] = v.y; }
+#pragma unroll
+    for (int c = 0; c < 8; c++) {
+#pragma unroll
+      for (int s = 0; s < 8; s++) {
+        const int ex = ((c >> 2) & 1) - ((s >> 2) & 1);
+        const int ey = ((c >> 1) & 1) - ((s >> 1) & 1);
+        const int ez = (c & 1) - (s & 1);
+        const double2 t = To[ex + 7 * ey + 49 * ez];
+        yr[s] = __builtin_fma(t.x, xr[c], __builtin_fma(-t.y, xi[c], yr[s]));
+        yi[s] = __builtin_fma(t.x, xi[c], __builtin_fma(t.y, xr[c], yi[s]));
+      }
+    }
+  }
+  if (p < nparents) {
+    double2* yb = Y + ((size_t)yslot[p] * F + f) * 8;
+#pragma unroll
+    for (int s = 0; s < 8; s++) yb[s] = make_double2(yr[s], yi[s]);
+  }
+}
+
+/* Inverse transform of the 8 sibling spectra of one child block, pruned to
+ * the NP^3 cube holding the check surface; normalised by 1/N^3 and added
+ * into the downward check values (compute_ifft_trg, fmm/omp_sse_block.cc:482-533:
+ * FFTW c2r semantics, the kx = 0 and kx = N/2 imaginary parts are ignored). */
+template <int NP>
+__global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks, const int* __restrict__ yslot,
+                                                 const double2* __restrict__ Y, const double2* __restrict__ tw,
+                                                 const int* __restrict__ surf2cube, /* NE: (ix*NP+iy)*NP+iz */
+                                                 double* __restrict__ dchk, int ld_e) {
+  using G = Grid<NP>;
+  constexpr int N = G::N, H = G::H, F = G::F, NE = G::NE, OFF = G::OFF;
+  __shared__ double2 B[8][NP][N][H]; /* [oct][iz][ky][kx] */
+  __shared__ double2 A[8][NP][NP][H]; /* [oct][iz][iy][kx] */
+  __shared__ double2 w[N];
+  const int tid = threadIdx.x;
+  const int first = blocks[blockIdx.x];
+  if (tid < N) w[tid] = tw[tid];
+  __syncthreads();
+  const double2* in = Y + (size_t)yslot[blockIdx.x] * F * 8;
+  /* z pass: thread per (ky, kx, oct), NP outputs in registers */
+  for (int e = tid; e < N * H * 8; e += blockDim.x) {
+    const int o = e & 7, kk = e >> 3; /* kk = ky*H + kx */
+    double re[NP], im[NP];
+#pragma unroll
+    for (int iz = 0; iz < NP; iz++) { re[iz] = 0.0; im[iz] = 0.0; }
+    for (int kz = 0; kz < N; kz++) {
+      const double2 v = in[(size_t)(kz * N * H + kk) * 8 + o];
+#pragma unroll
+      for (int iz = 0; iz < NP; iz++) {
+        const double2 t = w[(kz * (iz + OFF)) % N]; /* conj applied below */
+        re[iz] = __builtin_fma(v.x, t.x, __builtin_fma(v.y, t.y, re[iz]));
+        im[iz] = __builtin_fma(v.y, t.x, __builtin_fma(-v.x, t.y, im[iz]));
+      }
+    }
+    const int ky = kk / H, kx = kk % H;
+#pragma unroll
+    for (int iz = 0; iz < NP; iz++) B[o][iz][ky][kx] = make_double2(re[iz], im[iz]);
+  }
+  __syncthreads();
+  /* y pass */
+  for (int e = tid; e < 8 * NP * NP * H; e += blockDim.x) {
+    const int kx = e % H, iy = (e / H) % NP, iz = (e / (H * NP)) % NP, o = e / (H * NP * NP);
+    double re = 0.0, im = 0.0;
+    for (int ky = 0; ky < N; ky++) {
+      const double2 v = B[o][iz][ky][kx];
+      const double2 t = w[(ky * (iy + OFF)) % N];
+      re = __builtin_fma(v.x, t.x, __builtin_fma(v.y, t.y, re));
+      im = __builtin_fma(v.y, t.x, __builtin_fma(-v.x, t.y, im));
+    }
+    A[o][iz][iy][kx] = make_double2(re, im);
+  }
+  __syncthreads();
+  /* x pass (c2r) at the surface samples */
+  const double nrm = 1.0 / (double)(N * N * N);
+  for (int e = tid; e < 8 * NE; e += blockDim.x) {
+    const int o = e / NE, j = e % NE;
+    const int cube = surf2cube[j];
+    const int ix = cube / (NP * NP), iy = (cube / NP) % NP, iz = cube % NP;
+    const double2* r = A[o][iz][iy];
+    const int xg = ix + OFF;
+    double s = r[0].x + ((xg & 1) ? -r[N / 2].x : r[N / 2].x);
+    double acc = 0.0;
+    for (int kx = 1; kx < N / 2; kx++) {
+      const double2 t = w[(kx * xg) % N];
+      acc = __builtin_fma(r[kx].x, t.x, __builtin_fma(r[kx].y, t.y, acc));
+    }
+    s = __builtin_fma(2.0, acc, s);
+    dchk[(size_t)(first + o) * ld_e + j] += nrm * s;
+  }
+}
+
+/* ------------------------------------------------- W, X and L2T evaluation */
+
+/* Adds 1/4pi * (field of the staged tile) to `ntrg` target points of a
+ * leaf (trg4 + tbeg, out = pot + tbeg) or of a surface (surf != nullptr:
+ * c + r * SP, out = check values). */
+template <bool MASK>
+__device__ __forceinline__ void apply_tile(const double4* tile, int ns, const double4* __restrict__ trg4, int tbeg,
+                                           int ntrg, const double* __restrict__ surf, int ld_surf, double4 g,
+                                           double* __restrict__ out, int lane, int wave, int nwaves) {
+  const int ngroups = (ntrg + 7) / 8;
+  for (int grp = wave; grp < ngroups; grp += nwaves) {
+    double tx[8], ty[8], tz[8], acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) {
+      const int j = min(grp * 8 + t, ntrg - 1);
+      if (surf) {
+        tx[t] = g.x + g.w * surf[j];
+        ty[t] = g.y + g.w * surf[ld_surf + j];
+        tz[t] = g.z + g.w * surf[2 * ld_surf + j];
+      } else {
+        const double4 q = trg4[tbeg + j];
+        tx[t] = q.x; ty[t] = q.y; tz[t] = q.z;
+      }
+      acc[t] = 0.0;
+    }
+    eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
+    const double sum = wave_reduce8(acc, lane);
+    const int j = grp * 8 + (lane >> 3);
+    if ((lane & 7) == 0 && j < ntrg) out[j] += OOFP_R * sum;
+  }
+}
+
+/* W-list (wlist_eval__gpu, fmm/cuda.cu:2070-2197): one block per target leaf. */
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_wlist(const int* __restrict__ leaves, const int* __restrict__ wptr,
+                                               const int* __restrict__ widx, const int* __restrict__ child,
+                                               const int* __restrict__ snum, const int* __restrict__ sbeg,
+                                               const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                               const double4* __restrict__ geo, const double* __restrict__ surf_ue,
+                                               int ld_surf, int n_e, const double* __restrict__ uden, int ld_e,
+                                               const double4* __restrict__ src4, const double4* __restrict__ trg4,
+                                               double* __restrict__ pot) {
+  __shared__ double4 tile[1024];
+  const int t = leaves[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int k = wptr[t]; k < wptr[t + 1]; k++) {
+    const int s = widx[k];
+    const bool direct = child[s] == -1 && snum[s] < n_e;
+    const int n = direct ? snum[s] : n_e;
+    for (int base = 0; base < n; base += 1024) {
+      __syncthreads();
+      const int ns = direct ? stage_sources(tile, 1024, src4, sbeg[s] + base, n - base, nullptr, 0, geo[s], nullptr, tid, 256)
+                            : stage_sources(tile, 1024, nullptr, 0, n, surf_ue, ld_surf, geo[s], uden + (size_t)s * ld_e, tid, 256);
+      __syncthreads();
+      apply_tile<MASK>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+    }
+  }
+}
+
+/* X-list (xlist_eval__gpu, fmm/cuda.cu:2380-2493): one block per target node. */
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_xlist(const int* __restrict__ nodes, const int* __restrict__ xptr,
+                                               const int* __restrict__ xidx, const int* __restrict__ child,
+                                               const int* __restrict__ snum, const int* __restrict__ sbeg,
+                                               const int* __restrict__ tbeg, const int* __restrict__ tnum,
+                                               const double4* __restrict__ geo, const double* __restrict__ surf_dc,
+                                               int ld_surf, int n_e, double* __restrict__ dchk, int ld_e,
+                                               const double4* __restrict__ src4, const double4* __restrict__ trg4,
+                                               double* __restrict__ pot) {
+  __shared__ double4 tile[1024];
+  const int t = nodes[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const bool direct = child[t] == -1 && tnum[t] < n_e;
+  for (int k = xptr[t]; k < xptr[t + 1]; k++) {
+    const int s = xidx[k];
+    for (int base = 0; base < snum[s]; base += 1024) {
+      __syncthreads();
+      const int ns = stage_sources(tile, 1024, src4, sbeg[s] + base, snum[s] - base, nullptr, 0, geo[s], nullptr, tid, 256);
+      __syncthreads();
+      if (direct)
+        apply_tile<MASK>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+      else
+        apply_tile<false>(tile, ns, nullptr, 0, n_e, surf_dc, ld_surf, geo[t], dchk + (size_t)t * ld_e, lane, wave, 4);
+    }
+  }
+}
+
+/* L2T (down_eval__gpu_leaves, fmm/cuda.cu:1804-1872): one block per target leaf. */
+static __global__ __launch_bounds__(256) void k_l2t(const int* __restrict__ leaves, const int* __restrict__ tbeg,
+                                             const int* __restrict__ tnum, const double4* __restrict__ geo,
+                                             const double* __restrict__ surf_de, int ld_surf, int n_e,
+                                             const double* __restrict__ dden, int ld_e,
+                                             const double4* __restrict__ trg4, double* __restrict__ pot) {
+  __shared__ double4 tile[320];
+  const int t = leaves[blockIdx.x];
+  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ns = stage_sources(tile, 320, nullptr, 0, n_e, surf_de, ld_surf, geo[t], dden + (size_t)t * ld_e, tid, 256);
+  __syncthreads();
+  apply_tile<false>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+}
+
+static __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace kifmm_dev

@@ -1,0 +1,856 @@
+/*
+ * kifmm_gpu.hip -- create / evaluate / destroy of the MI355X KIFMM evaluator.
+ *
+ * kifmm_gpu_create is the counterpart of preproc() (fmm/evaluate--cuda.cu:1441-2239):
+ * it derives the device work lists from the flattened tree and uploads
+ * everything once.  kifmm_gpu_evaluate is the counterpart of run()
+ * (fmm/run--cuda.cu:27-345): up (S2M + M2M), U, V (forward DFT,
+ * translation spectra, spectral multiply-add, inverse DFT), W, X, down
+ * (L2L + DC2DE + L2T) and the copy back, in that order, on one stream.
+ *
+ * Device layout (HBM):
+ *   src4 / trg4   gathered points as (x, y, z, w) double4 records
+ *   uden/dchk/dden  node-major vectors, ld_e = n_e rounded up to 8 doubles,
+ *                 zero padding
+ *   chk           UC check potentials of one batch of columns (ld_uc)
+ *   X / Y         source / target spectra, slot-major [slot][f][octant]:
+ *                 slot 1 + b holds the 8 children of child block b (nodes
+ *                 1+8b .. 8+8b); slot 0 is all zero and stands for a missing
+ *                 neighbour
+ *   T             translation spectra, frequency-major [f][343]
+ *   operators     padded column-major, UE2UC and DE2DC stacked by octant
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "kifmm_hip.h"
+#include "../host/kifmm_internal.h"
+#include "kernels.hpp"
+
+using namespace kifmm_dev;
+using kifmm::set_error;
+
+namespace {
+
+inline int rup(int x, int m) { return (x + m - 1) / m * m; }
+
+struct GemmList {
+  int n = 0;
+  int* in_off = nullptr;
+  int* out_off = nullptr;
+  int* sexp = nullptr;
+};
+
+struct LevelOps {
+  GemmList m2m_a, m2m_b; /* UE2UC stack -> chk, UC2UE -> uden */
+  GemmList l2l, dc2de;
+};
+
+}  // namespace
+
+struct kifmm_gpu {
+  int dev = 0;
+  hipStream_t stream = nullptr;
+  hipEvent_t ev[16] = {};
+  int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
+  int N = 0, H = 0, F = 0;
+  int n_nodes = 0, n_src = 0, n_trg = 0, coincident = 0, max_depth = 0, precision = 0;
+  std::vector<void*> allocs;
+  /* device data */
+  double4 *src4 = nullptr, *trg4 = nullptr, *geo = nullptr;
+  double *pot = nullptr, *uden = nullptr, *dchk = nullptr, *dden = nullptr, *chk = nullptr;
+  double *UC2UE = nullptr, *UE2UC = nullptr, *DC2DE = nullptr, *DE2DC = nullptr;
+  double *surf_ue = nullptr, *surf_uc = nullptr, *surf_de = nullptr, *surf_dc = nullptr;
+  double2 *X = nullptr, *Y = nullptr, *T = nullptr, *tw = nullptr;
+  int *cube2surf = nullptr, *surf2cube = nullptr;
+  int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *tnum = nullptr, *depth = nullptr;
+  int *wptr = nullptr, *widx = nullptr, *xptr = nullptr, *xidx = nullptr;
+  /* work lists */
+  int4* p2p_items = nullptr; int n_p2p = 0;
+  int* rng_ptr = nullptr; int2* rng = nullptr;
+  int4* s2m_items = nullptr; int n_s2m = 0;
+  GemmList s2m_gemm;
+  std::vector<LevelOps> lv;
+  int* vblocks = nullptr; int* vslot = nullptr; int n_vblocks = 0;
+  int* hnbr = nullptr; int* hslot = nullptr; int n_hpar = 0;
+  int* wleaves = nullptr; int n_wleaves = 0;
+  int* xnodes = nullptr; int n_xnodes = 0;
+  int* l2t_leaves = nullptr; int n_l2t = 0;
+  size_t chk_cols = 0;
+  kifmm_work_counts counts{};
+};
+
+namespace {
+
+#define HIP_TRY(expr)                                                                      \
+  do {                                                                                     \
+    hipError_t e_ = (expr);                                                                \
+    if (e_ != hipSuccess) {                                                                \
+      set_error(std::string(#expr) + ": " + hipGetErrorString(e_));                        \
+      return KIFMM_EDEVICE;                                                                \
+    }                                                                                      \
+  } while (0)
+
+template <typename T>
+int dev_upload(kifmm_gpu* g, T** dst, const T* src, size_t n) {
+  *dst = nullptr;
+  const size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+  void* p = nullptr;
+  if (hipMalloc(&p, bytes) != hipSuccess) {
+    set_error("hipMalloc of " + std::to_string(bytes) + " bytes failed");
+    return KIFMM_ENOMEM;
+  }
+  g->allocs.push_back(p);
+  *dst = (T*)p;
+  if (src && n) HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
+  else HIP_TRY(hipMemset(p, 0, bytes));
+  return KIFMM_OK;
+}
+
+template <typename T>
+int dev_zero(kifmm_gpu* g, T** dst, size_t n) {
+  return dev_upload<T>(g, dst, nullptr, n);
+}
+
+#define TRY(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != KIFMM_OK) return r_; \
+  } while (0)
+
+/* boundary nodes of an np^3 grid on [-R,R]^3, x slowest (fmm/trans.cc:91-120) */
+void host_surface(int np, double R, std::vector<double>& xyz, int ld) {
+  const double step = 2.0 / (np - 1);
+  int cnt = 0;
+  for (int i = 0; i < np; i++)
+    for (int j = 0; j < np; j++)
+      for (int k = 0; k < np; k++) {
+        if (!(i == 0 || i == np - 1 || j == 0 || j == np - 1 || k == 0 || k == np - 1)) continue;
+        xyz[cnt] = R * (-1.0 + i * step);
+        xyz[ld + cnt] = R * (-1.0 + j * step);
+        xyz[2 * ld + cnt] = R * (-1.0 + k * step);
+        cnt++;
+      }
+}
+
+int find_box(const kifmm_tree_view* t, int d, const int want[3]) {
+  int cur = 0;
+  while (t->depth[cur] < d && t->child[cur] != -1) {
+    const int sh = d - t->depth[cur] - 1;
+    const int o = (((want[0] >> sh) & 1) << 2) | (((want[1] >> sh) & 1) << 1) | ((want[2] >> sh) & 1);
+    cur = t->child[cur] + o;
+  }
+  return cur;
+}
+
+/* Structural checks of a caller-supplied tree view: every index a kernel
+ * derives from it must land inside the arrays it addresses. */
+int check_view(const kifmm_tree_view* t) {
+  const int nn = t->n_nodes;
+  auto bad = [](const std::string& m) {
+    set_error("kifmm_gpu_create: malformed tree view: " + m);
+    return KIFMM_EINVAL;
+  };
+  if (t->n_levels < 2 || t->level_start[0] != 0 || t->level_start[t->n_levels - 1] != nn)
+    return bad("level_start");
+  for (int i = 0; i < nn; i++) {
+    const int c = t->child[i];
+    if (c != -1 && (c < 1 || c + 8 > nn || (c - 1) % 8 != 0)) return bad("child index");
+    if (t->depth[i] < 0 || t->depth[i] > 30) return bad("depth");
+    if (t->depth[i] + 2 > t->n_levels) return bad("depth vs levels");
+    const int lim = 1 << t->depth[i];
+    for (int k = 0; k < 3; k++)
+      if (t->path[3 * i + k] < 0 || t->path[3 * i + k] >= lim) return bad("path");
+    if (t->src_num[i] < 0 || t->src_beg[i] < 0 || (long)t->src_beg[i] + t->src_num[i] > t->n_src) return bad("source range");
+    if (t->trg_num[i] < 0 || t->trg_beg[i] < 0 || (long)t->trg_beg[i] + t->trg_num[i] > t->n_trg) return bad("target range");
+    if ((t->src_num[i] > 0 || t->trg_num[i] > 0) && c != -1) return bad("points on an inner node");
+  }
+  for (int d = 0; d + 1 < t->n_levels; d++)
+    if (t->level_start[d] > t->level_start[d + 1]) return bad("level order");
+  const int* ptrs[4] = {t->u_ptr, t->v_ptr, t->w_ptr, t->x_ptr};
+  const int* idxs[4] = {t->u_idx, t->v_idx, t->w_idx, t->x_idx};
+  for (int L = 0; L < 4; L++) {
+    if (ptrs[L][0] != 0) return bad("list pointer");
+    for (int i = 0; i < nn; i++) {
+      if (ptrs[L][i + 1] < ptrs[L][i]) return bad("list pointer order");
+      for (int k = ptrs[L][i]; k < ptrs[L][i + 1]; k++)
+        if (idxs[L][k] < 0 || idxs[L][k] >= nn) return bad("list entry");
+    }
+  }
+  return KIFMM_OK;
+}
+
+/* Everything the device needs, derived on the host from the tree view.
+ * make_plan() touches no device; upload() copies a checked plan. */
+struct HostGemm {
+  std::vector<int> in, out, sexp;
+};
+struct HostLevel {
+  HostGemm m2m_a, m2m_b, l2l, dc2de;
+};
+struct Plan {
+  int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
+  int N = 0, H = 0, F = 0, nblk = 0, max_depth = 0;
+  std::vector<int4> p2p_items;
+  std::vector<int> rng_ptr;
+  std::vector<int2> rng;
+  std::vector<int4> s2m_items;
+  HostGemm s2m_gemm;
+  std::vector<HostLevel> lv;
+  size_t chk_cols = 0;
+  std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
+  kifmm_work_counts wc{};
+};
+
+bool offsets_ok(const std::vector<int>& off, long len_read, long buf_len) {
+  for (int o : off)
+    if (o < 0 || (long)o + len_read > buf_len) return false;
+  return true;
+}
+
+int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
+  TRY(check_view(t));
+  const int np = t->np;
+  P.np = np;
+  P.n_e = ops->n_e;
+  P.n_uc = ops->n_uc;
+  P.ld_e = rup(P.n_e, 8);
+  P.ld_uc = rup(P.n_uc, 8);
+  P.lda_e = rup(P.n_e, 16);
+  P.lda_uc = rup(P.n_uc, 16);
+  P.lda_l2l = rup(8 * P.ld_e, 16);
+  P.ld_surf = rup(std::max(P.n_uc, P.n_e), 8);
+  P.N = 2 * np;
+  P.H = np + 1;
+  P.F = P.N * P.N * P.H;
+  const int nn = t->n_nodes, ne = P.n_e, nuc = P.n_uc, lde = P.ld_e, lduc = P.ld_uc;
+  P.nblk = (nn - 1) / 8;
+  int maxd = 0;
+  for (int i = 0; i < nn; i++) maxd = std::max(maxd, t->depth[i]);
+  P.max_depth = maxd;
+  kifmm_work_counts& wc = P.wc;
+  std::memset(&wc, 0, sizeof wc);
+  auto fail = [](const char* m) {
+    set_error(std::string("internal: ") + m);
+    return KIFMM_EINVAL;
+  };
+
+  /* U-list: merged source ranges per target leaf, items of <= 32 targets */
+  P.rng_ptr.assign(1, 0);
+  for (int i = 0; i < nn; i++) {
+    if (t->child[i] != -1 || t->trg_num[i] <= 0) continue;
+    const int set = (int)P.rng_ptr.size() - 1;
+    for (int k = t->u_ptr[i]; k < t->u_ptr[i + 1]; k++) {
+      const int s = t->u_idx[k];
+      const int b = t->src_beg[s], n = t->src_num[s];
+      if (n <= 0) continue;
+      wc.u_pairs += (double)n * t->trg_num[i];
+      if ((int)P.rng.size() > P.rng_ptr.back() && P.rng.back().x + P.rng.back().y == b) P.rng.back().y += n;
+      else P.rng.push_back(make_int2(b, n));
+    }
+    P.rng_ptr.push_back((int)P.rng.size());
+    for (int c = 0; c < t->trg_num[i]; c += 32)
+      P.p2p_items.push_back(make_int4(set, t->trg_beg[i] + c, std::min(32, t->trg_num[i] - c), 0));
+  }
+  /* k_p2p_ulist reads trg4[it.y .. it.y+it.z-1] and src4 inside each range */
+  for (const int4& it : P.p2p_items)
+    if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > 32 || it.y < 0 ||
+        (long)it.y + it.z > t->n_trg)
+      return fail("U-list work item out of range");
+  for (const int2& r : P.rng)
+    if (r.x < 0 || r.y < 1 || (long)r.x + r.y > t->n_src) return fail("U-list source range out of range");
+
+  /* S2M + UC2UE over source leaves */
+  for (int i = 0; i < nn; i++) {
+    if (t->child[i] != -1 || !(t->tag[i] & KIFMM_SRCNODE) || t->src_num[i] <= 0) continue;
+    const int col = (int)P.s2m_items.size();
+    P.s2m_items.push_back(make_int4(i, t->src_beg[i], t->src_num[i], col));
+    P.s2m_gemm.in.push_back(col * lduc);
+    P.s2m_gemm.out.push_back(i * lde);
+    P.s2m_gemm.sexp.push_back(-t->depth[i]);
+    wc.s2m_pairs += (double)nuc * t->src_num[i];
+    wc.gemm_flops += 2.0 * ne * nuc;
+  }
+  P.chk_cols = P.s2m_items.size();
+  for (const int4& it : P.s2m_items)
+    if (it.y < 0 || it.z < 1 || (long)it.y + it.z > t->n_src) return fail("S2M source range out of range");
+
+  /* per-level M2M, L2L and DC2DE column lists */
+  P.lv.assign(maxd + 1, HostLevel());
+  for (int d = 0; d <= maxd; d++) {
+    HostLevel& L = P.lv[d];
+    int col = 0;
+    for (int i = t->level_start[d]; i < t->level_start[d + 1]; i++) {
+      if (t->depth[i] != d) return fail("level_start does not match depth");
+      if (t->child[i] != -1 && (t->tag[i] & KIFMM_SRCNODE)) {
+        L.m2m_a.in.push_back(t->child[i] * lde); L.m2m_a.out.push_back(col * lduc); L.m2m_a.sexp.push_back(d + 1);
+        L.m2m_b.in.push_back(col * lduc); L.m2m_b.out.push_back(i * lde); L.m2m_b.sexp.push_back(-d);
+        col++;
+        wc.gemm_flops += 2.0 * nuc * 8 * ne + 2.0 * ne * nuc;
+      }
+      if (d + 1 >= 3 && t->child[i] != -1 && (t->tag[i] & KIFMM_TRGNODE)) {
+        L.l2l.in.push_back(i * lde); L.l2l.out.push_back(t->child[i] * lde); L.l2l.sexp.push_back(d);
+        wc.gemm_flops += 2.0 * 8 * ne * ne;
+      }
+      if (d >= 2 && (t->tag[i] & KIFMM_TRGNODE)) {
+        L.dc2de.in.push_back(i * lde); L.dc2de.out.push_back(i * lde); L.dc2de.sexp.push_back(-d);
+        wc.gemm_flops += 2.0 * ne * ne;
+      }
+    }
+    P.chk_cols = std::max(P.chk_cols, (size_t)col);
+  }
+  /* GEMM columns: inputs read K doubles, outputs write M doubles */
+  {
+    const long nodes = (long)nn * lde, chk = (long)std::max<size_t>(P.chk_cols, 1) * lduc;
+    bool ok = offsets_ok(P.s2m_gemm.in, lduc, chk) && offsets_ok(P.s2m_gemm.out, ne, nodes);
+    for (const HostLevel& L : P.lv)
+      ok = ok && offsets_ok(L.m2m_a.in, 8L * lde, nodes) && offsets_ok(L.m2m_a.out, nuc, chk) &&
+           offsets_ok(L.m2m_b.in, lduc, chk) && offsets_ok(L.m2m_b.out, ne, nodes) &&
+           offsets_ok(L.l2l.in, lde, nodes) && offsets_ok(L.l2l.out, 8L * lde, nodes) &&
+           offsets_ok(L.dc2de.in, lde, nodes) && offsets_ok(L.dc2de.out, ne, nodes);
+    if (!ok) return fail("GEMM column offset out of range");
+  }
+
+  /* V-list: child blocks below parents of depth >= 1 and their 27 neighbour slots */
+  for (int i = 0; i < nn; i++) {
+    if (t->child[i] == -1 || t->depth[i] < 1) continue;
+    const int blk = (t->child[i] - 1) / 8;
+    P.vblocks.push_back(t->child[i]);
+    P.vslot.push_back(blk + 1);
+    P.hslot.push_back(blk + 1);
+    const int d = t->depth[i], lim = 1 << d;
+    for (int o = 0; o < 27; o++) {
+      const int off[3] = {o / 9 - 1, (o / 3) % 3 - 1, o % 3 - 1};
+      int w[3], slot = 0;
+      bool in = true;
+      for (int k = 0; k < 3; k++) {
+        w[k] = t->path[3 * i + k] + off[k];
+        in = in && w[k] >= 0 && w[k] < lim;
+      }
+      if (in) {
+        const int c = find_box(t, d, w);
+        if (t->depth[c] == d && t->child[c] != -1) slot = (t->child[c] - 1) / 8 + 1;
+      }
+      P.hnbr.push_back(slot);
+    }
+  }
+  for (size_t k = 0; k < P.vblocks.size(); k++)
+    if (P.vblocks[k] < 1 || P.vblocks[k] + 8 > nn || P.vslot[k] < 1 || P.vslot[k] > P.nblk || P.hslot[k] < 1 ||
+        P.hslot[k] > P.nblk)
+      return fail("V-list block out of range");
+  for (int v : P.hnbr)
+    if (v < 0 || v > P.nblk) return fail("V-list neighbour slot out of range");
+  wc.v_entries = t->v_ptr[nn];
+  wc.v_cmac_ref = 0.5 * (2.0 * P.F) * wc.v_entries;
+  wc.v_cmac = 26.0 * 64.0 * P.F * (double)P.hslot.size();
+  wc.fft_bytes = (double)P.vblocks.size() * 8 * (2.0 * 8 * ne + 2.0 * 16 * P.F);
+  wc.hadamard_bytes = (double)P.hslot.size() * 8 * 16.0 * P.F * 2 + 343.0 * 16 * P.F;
+
+  /* W, X and L2T work */
+  for (int i = 0; i < nn; i++) {
+    const bool leaf = t->child[i] == -1;
+    const bool trg = (t->tag[i] & KIFMM_TRGNODE) != 0;
+    if (trg && leaf && t->w_ptr[i + 1] > t->w_ptr[i]) {
+      P.wleaves.push_back(i);
+      for (int k = t->w_ptr[i]; k < t->w_ptr[i + 1]; k++) {
+        const int s = t->w_idx[k];
+        const bool direct = t->child[s] == -1 && t->src_num[s] < ne;
+        wc.w_pairs += (double)t->trg_num[i] * (direct ? t->src_num[s] : ne);
+      }
+    }
+    if (trg && t->x_ptr[i + 1] > t->x_ptr[i]) {
+      P.xnodes.push_back(i);
+      const bool direct = leaf && t->trg_num[i] < ne;
+      for (int k = t->x_ptr[i]; k < t->x_ptr[i + 1]; k++)
+        wc.x_pairs += (double)t->src_num[t->x_idx[k]] * (direct ? t->trg_num[i] : ne);
+    }
+    if (trg && leaf && t->trg_num[i] > 0 && t->depth[i] >= 2) {
+      P.l2t.push_back(i);
+      wc.l2t_pairs += (double)ne * t->trg_num[i];
+    }
+  }
+  /* apply_tile clamps target indices to [tbeg, tbeg + tnum - 1]: the leaves
+   * handed to k_wlist / k_xlist (direct) / k_l2t must own a target */
+  for (int i : P.wleaves)
+    if (t->trg_num[i] < 1) return fail("W-list leaf without targets");
+  for (int i : P.xnodes)
+    if (t->child[i] == -1 && t->trg_num[i] < ne && t->trg_num[i] < 1) return fail("X-list leaf without targets");
+  for (int i : P.l2t)
+    if (t->trg_num[i] < 1) return fail("L2T leaf without targets");
+  if (ne > 320) return fail("NP too large for the L2T tile");
+  return KIFMM_OK;
+}
+
+int upload_gemm(kifmm_gpu* g, GemmList& L, const HostGemm& h) {
+  L.n = (int)h.in.size();
+  TRY(dev_upload(g, &L.in_off, h.in.data(), h.in.size()));
+  TRY(dev_upload(g, &L.out_off, h.out.data(), h.out.size()));
+  TRY(dev_upload(g, &L.sexp, h.sexp.data(), h.sexp.size()));
+  return KIFMM_OK;
+}
+
+int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
+  Plan P;
+  TRY(make_plan(t, ops, P));
+  const int np = P.np, nn = t->n_nodes, ne = P.n_e, nuc = P.n_uc, lde = P.ld_e, lduc = P.ld_uc;
+  g->np = np; g->n_e = ne; g->n_uc = nuc; g->ld_e = lde; g->ld_uc = lduc;
+  g->lda_e = P.lda_e; g->lda_uc = P.lda_uc; g->lda_l2l = P.lda_l2l; g->ld_surf = P.ld_surf;
+  g->N = P.N; g->H = P.H; g->F = P.F;
+  g->n_nodes = nn; g->n_src = t->n_src; g->n_trg = t->n_trg;
+  g->coincident = t->coincident;
+  g->max_depth = P.max_depth;
+  g->counts = P.wc;
+
+  /* points */
+  {
+    std::vector<double4> s(t->n_src), q(t->n_trg);
+    for (int i = 0; i < t->n_src; i++) s[i] = make_double4(t->sx[i], t->sy[i], t->sz[i], t->sden[i]);
+    for (int i = 0; i < t->n_trg; i++) q[i] = make_double4(t->tx[i], t->ty[i], t->tz[i], 0.0);
+    TRY(dev_upload(g, &g->src4, s.data(), s.size()));
+    TRY(dev_upload(g, &g->trg4, q.data(), q.size()));
+    TRY(dev_zero(g, &g->pot, (size_t)t->n_trg));
+  }
+  /* node geometry (fmm/input.cc:113-137) and node arrays */
+  {
+    std::vector<double4> geo(nn);
+    for (int i = 0; i < nn; i++) {
+      const double sc = (double)(1 << t->depth[i]);
+      geo[i] = make_double4(-1.0 + (2.0 * 1.0) * (t->path[3 * i] + 0.5) / sc,
+                            -1.0 + (2.0 * 1.0) * (t->path[3 * i + 1] + 0.5) / sc,
+                            -1.0 + (2.0 * 1.0) * (t->path[3 * i + 2] + 0.5) / sc, 1.0 / sc);
+    }
+    TRY(dev_upload(g, &g->geo, geo.data(), geo.size()));
+    TRY(dev_upload(g, &g->child, t->child, nn));
+    TRY(dev_upload(g, &g->snum, t->src_num, nn));
+    TRY(dev_upload(g, &g->sbeg, t->src_beg, nn));
+    TRY(dev_upload(g, &g->tbeg, t->trg_beg, nn));
+    TRY(dev_upload(g, &g->tnum, t->trg_num, nn));
+    TRY(dev_upload(g, &g->depth, t->depth, nn));
+    TRY(dev_upload(g, &g->wptr, t->w_ptr, nn + 1));
+    TRY(dev_upload(g, &g->widx, t->w_idx, t->w_ptr[nn]));
+    TRY(dev_upload(g, &g->xptr, t->x_ptr, nn + 1));
+    TRY(dev_upload(g, &g->xidx, t->x_idx, t->x_ptr[nn]));
+    TRY(dev_zero(g, &g->uden, (size_t)nn * lde));
+    TRY(dev_zero(g, &g->dchk, (size_t)nn * lde));
+    TRY(dev_zero(g, &g->dden, (size_t)nn * lde));
+  }
+  /* operators, padded column-major */
+  {
+    std::vector<double> a((size_t)g->lda_e * lduc, 0.0);
+    for (int k = 0; k < nuc; k++)
+      for (int i = 0; i < ne; i++) a[(size_t)k * g->lda_e + i] = ops->uc2ue[(size_t)k * ne + i];
+    TRY(dev_upload(g, &g->UC2UE, a.data(), a.size()));
+    std::vector<double> b((size_t)g->lda_uc * 8 * lde, 0.0);
+    for (int o = 0; o < 8; o++)
+      for (int k = 0; k < ne; k++)
+        for (int i = 0; i < nuc; i++)
+          b[(size_t)(o * lde + k) * g->lda_uc + i] = ops->ue2uc[(size_t)o * nuc * ne + (size_t)k * nuc + i];
+    TRY(dev_upload(g, &g->UE2UC, b.data(), b.size()));
+    std::vector<double> c((size_t)g->lda_e * lde, 0.0);
+    for (int k = 0; k < ne; k++)
+      for (int i = 0; i < ne; i++) c[(size_t)k * g->lda_e + i] = ops->dc2de[(size_t)k * ne + i];
+    TRY(dev_upload(g, &g->DC2DE, c.data(), c.size()));
+    std::vector<double> d((size_t)g->lda_l2l * lde, 0.0);
+    for (int o = 0; o < 8; o++)
+      for (int k = 0; k < ne; k++)
+        for (int i = 0; i < ne; i++)
+          d[(size_t)k * g->lda_l2l + o * lde + i] = ops->de2dc[(size_t)o * ne * ne + (size_t)k * ne + i];
+    TRY(dev_upload(g, &g->DE2DC, d.data(), d.size()));
+  }
+  /* unit surfaces (fmm/trans.cc:73-86) and DFT tables */
+  {
+    const int ls = g->ld_surf;
+    std::vector<double> s(3 * (size_t)ls, 0.0);
+    host_surface(np, 1.0, s, ls); TRY(dev_upload(g, &g->surf_ue, s.data(), s.size()));
+    std::fill(s.begin(), s.end(), 0.0);
+    host_surface(np + 2, 3.0, s, ls); TRY(dev_upload(g, &g->surf_uc, s.data(), s.size()));
+    std::fill(s.begin(), s.end(), 0.0);
+    host_surface(np, 3.0, s, ls); TRY(dev_upload(g, &g->surf_de, s.data(), s.size()));
+    std::fill(s.begin(), s.end(), 0.0);
+    host_surface(np, 1.0, s, ls); TRY(dev_upload(g, &g->surf_dc, s.data(), s.size()));
+    const int N = g->N;
+    std::vector<double2> w(N);
+    for (int m = 0; m < N; m++) {
+      const long double a = 2.0L * 3.141592653589793238462643383279502884L * (long double)m / (long double)N;
+      w[m] = make_double2((double)cosl(a), (double)(-sinl(a)));
+    }
+    TRY(dev_upload(g, &g->tw, w.data(), w.size()));
+    std::vector<int> c2s(np * np * np, -1), s2c;
+    for (int i = 0; i < np; i++)
+      for (int j = 0; j < np; j++)
+        for (int k = 0; k < np; k++)
+          if (i == 0 || i == np - 1 || j == 0 || j == np - 1 || k == 0 || k == np - 1) {
+            c2s[(i * np + j) * np + k] = (int)s2c.size();
+            s2c.push_back((i * np + j) * np + k);
+          }
+    TRY(dev_upload(g, &g->cube2surf, c2s.data(), c2s.size()));
+    TRY(dev_upload(g, &g->surf2cube, s2c.data(), s2c.size()));
+  }
+  /* work lists */
+  g->n_p2p = (int)P.p2p_items.size();
+  TRY(dev_upload(g, &g->p2p_items, P.p2p_items.data(), P.p2p_items.size()));
+  TRY(dev_upload(g, &g->rng_ptr, P.rng_ptr.data(), P.rng_ptr.size()));
+  TRY(dev_upload(g, &g->rng, P.rng.data(), P.rng.size()));
+  g->n_s2m = (int)P.s2m_items.size();
+  TRY(dev_upload(g, &g->s2m_items, P.s2m_items.data(), P.s2m_items.size()));
+  TRY(upload_gemm(g, g->s2m_gemm, P.s2m_gemm));
+  g->lv.assign(P.max_depth + 1, LevelOps());
+  for (int d = 0; d <= P.max_depth; d++) {
+    TRY(upload_gemm(g, g->lv[d].m2m_a, P.lv[d].m2m_a));
+    TRY(upload_gemm(g, g->lv[d].m2m_b, P.lv[d].m2m_b));
+    TRY(upload_gemm(g, g->lv[d].l2l, P.lv[d].l2l));
+    TRY(upload_gemm(g, g->lv[d].dc2de, P.lv[d].dc2de));
+  }
+  g->chk_cols = P.chk_cols;
+  TRY(dev_zero(g, &g->chk, std::max<size_t>(P.chk_cols, 1) * lduc));
+  g->n_vblocks = (int)P.vblocks.size();
+  g->n_hpar = (int)P.hslot.size();
+  TRY(dev_upload(g, &g->vblocks, P.vblocks.data(), P.vblocks.size()));
+  TRY(dev_upload(g, &g->vslot, P.vslot.data(), P.vslot.size()));
+  TRY(dev_upload(g, &g->hnbr, P.hnbr.data(), P.hnbr.size()));
+  TRY(dev_upload(g, &g->hslot, P.hslot.data(), P.hslot.size()));
+  const size_t spec = (size_t)(P.nblk + 1) * g->F * 8;
+  TRY(dev_zero(g, &g->X, spec));
+  TRY(dev_zero(g, &g->Y, spec));
+  TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
+  g->n_wleaves = (int)P.wleaves.size();
+  g->n_xnodes = (int)P.xnodes.size();
+  g->n_l2t = (int)P.l2t.size();
+  TRY(dev_upload(g, &g->wleaves, P.wleaves.data(), P.wleaves.size()));
+  TRY(dev_upload(g, &g->xnodes, P.xnodes.data(), P.xnodes.size()));
+  TRY(dev_upload(g, &g->l2t_leaves, P.l2t.data(), P.l2t.size()));
+  return KIFMM_OK;
+}
+
+#define LAUNCH_CHECK()                                                           \
+  do {                                                                           \
+    hipError_t e_ = hipGetLastError();                                           \
+    if (e_ != hipSuccess) {                                                      \
+      set_error(std::string("kernel launch: ") + hipGetErrorString(e_));         \
+      return KIFMM_EDEVICE;                                                      \
+    }                                                                            \
+  } while (0)
+
+int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda, const double* in, double* out,
+         int accumulate) {
+  if (L.n == 0) return KIFMM_OK;
+  constexpr int RT = 4;
+  const int nrt = (M + 15) / 16;
+  dim3 grid((L.n + 15) / 16, (nrt + 4 * RT - 1) / (4 * RT));
+  hipLaunchKernelGGL(k_gemm_cols<RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
+                     L.sexp, in, out, accumulate);
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+int fill(kifmm_gpu* g, double* p, size_t n) {
+  const int blocks = (int)std::min<size_t>(4096, (n + 255) / 256);
+  hipLaunchKernelGGL(k_fill, dim3(std::max(blocks, 1)), dim3(256), 0, g->stream, p, n, 0.0);
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+template <int NP>
+int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
+  if (g->n_vblocks > 0)
+    hipLaunchKernelGGL(k_dft_fwd<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->depth,
+                       g->uden, g->ld_e, g->tw, g->cube2surf, g->X);
+  LAUNCH_CHECK();
+  HIP_TRY(hipEventRecord(e1, g->stream));
+  hipLaunchKernelGGL(k_trans<NP>, dim3(343), dim3(256), 0, g->stream, g->tw, g->T);
+  LAUNCH_CHECK();
+  HIP_TRY(hipEventRecord(e2, g->stream));
+  if (g->n_hpar > 0)
+    hipLaunchKernelGGL(k_hadamard<NP>, dim3((g->n_hpar + 63) / 64, g->F), dim3(64), 0, g->stream, g->n_hpar,
+                       g->hnbr, g->hslot, g->T, g->X, g->Y);
+  LAUNCH_CHECK();
+  HIP_TRY(hipEventRecord(e3, g->stream));
+  if (g->n_vblocks > 0)
+    hipLaunchKernelGGL(k_dft_inv<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->Y,
+                       g->tw, g->surf2cube, g->dchk, g->ld_e);
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+template <bool MASK>
+int point_phases_u(kifmm_gpu* g) {
+  if (g->n_p2p > 0)
+    hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(g->n_p2p), dim3(256), 0, g->stream, g->p2p_items, g->rng_ptr, g->rng,
+                       g->src4, g->trg4, g->pot);
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+template <bool MASK>
+int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
+  if ((mask & KIFMM_PHASE_W) && g->n_wleaves > 0)
+    hipLaunchKernelGGL(k_wlist<MASK>, dim3(g->n_wleaves), dim3(256), 0, g->stream, g->wleaves, g->wptr, g->widx,
+                       g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_ue, g->ld_surf, g->n_e, g->uden,
+                       g->ld_e, g->src4, g->trg4, g->pot);
+  LAUNCH_CHECK();
+  HIP_TRY(hipEventRecord(ew, g->stream));
+  if ((mask & KIFMM_PHASE_X) && g->n_xnodes > 0)
+    hipLaunchKernelGGL(k_xlist<MASK>, dim3(g->n_xnodes), dim3(256), 0, g->stream, g->xnodes, g->xptr, g->xidx,
+                       g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_dc, g->ld_surf, g->n_e, g->dchk,
+                       g->ld_e, g->src4, g->trg4, g->pot);
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* times) {
+  hipEvent_t* ev = g->ev;
+  const int lde = g->ld_e, lduc = g->ld_uc;
+  HIP_TRY(hipSetDevice(g->dev));
+  HIP_TRY(hipEventRecord(ev[0], g->stream));
+  /* zero the accumulators (a fill kernel, so the sequence stays capturable) */
+  TRY(fill(g, g->pot, (size_t)g->n_trg));
+  TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
+  TRY(fill(g, g->dchk, (size_t)g->n_nodes * lde));
+  TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
+  /* up: S2M check potentials, UC2UE, then M2M bottom-up */
+  if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
+    if (g->coincident)
+      hipLaunchKernelGGL(k_s2m<false>, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->surf_uc,
+                         g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    else
+      hipLaunchKernelGGL(k_s2m<false>, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->surf_uc,
+                         g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    LAUNCH_CHECK();
+    TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
+  }
+  for (int d = g->max_depth - 1; d >= 0 && (mask & KIFMM_PHASE_UP); d--) {
+    TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
+    TRY(gemm(g, g->lv[d].m2m_b, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
+  }
+  HIP_TRY(hipEventRecord(ev[1], g->stream));
+  /* U-list */
+  if (mask & KIFMM_PHASE_U) TRY(g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g));
+  HIP_TRY(hipEventRecord(ev[2], g->stream));
+  /* V-list */
+  if (mask & KIFMM_PHASE_V) switch (g->np) {
+    case 3: TRY(vlist_phase<3>(g, ev[3], ev[4], ev[5])); break;
+    case 4: TRY(vlist_phase<4>(g, ev[3], ev[4], ev[5])); break;
+    case 6: TRY(vlist_phase<6>(g, ev[3], ev[4], ev[5])); break;
+    default: set_error("NP must be 3, 4 or 6 on the GPU"); return KIFMM_EUNSUP;
+  }
+  else {
+    HIP_TRY(hipEventRecord(ev[3], g->stream));
+    HIP_TRY(hipEventRecord(ev[4], g->stream));
+    HIP_TRY(hipEventRecord(ev[5], g->stream));
+  }
+  HIP_TRY(hipEventRecord(ev[6], g->stream));
+  /* W and X */
+  if (mask & (KIFMM_PHASE_W | KIFMM_PHASE_X))
+    TRY(g->coincident ? point_phases_wx<true>(g, mask, ev[7]) : point_phases_wx<false>(g, mask, ev[7]));
+  else
+    HIP_TRY(hipEventRecord(ev[7], g->stream));
+  HIP_TRY(hipEventRecord(ev[8], g->stream));
+  /* down: per level L2L (depth >= 3) then DC2DE (depth >= 2), then L2T */
+  for (int d = 2; d <= g->max_depth && (mask & KIFMM_PHASE_DOWN); d++) {
+    if (d >= 3)
+      TRY(gemm(g, g->lv[d - 1].l2l, 8 * lde, lde, g->DE2DC, g->lda_l2l, g->dden, g->dchk, 1));
+    TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
+  }
+  if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
+    hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(256), 0, g->stream, g->l2t_leaves, g->tbeg, g->tnum, g->geo,
+                       g->surf_de, g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
+  LAUNCH_CHECK();
+  HIP_TRY(hipEventRecord(ev[9], g->stream));
+  if (pot_leaf)
+    HIP_TRY(hipMemcpyAsync(pot_leaf, g->pot, sizeof(double) * g->n_trg, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipEventRecord(ev[10], g->stream));
+  HIP_TRY(hipEventSynchronize(ev[10]));
+  if (times) {
+    float ms[11] = {0};
+    for (int i = 1; i <= 10; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
+    float tot = 0;
+    HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev[10]));
+    std::memset(times, 0, sizeof *times);
+    times->up = ms[1];
+    times->ulist = ms[2];
+    times->p2p_kernel = ms[2];
+    times->v_fft_src = ms[3];
+    times->v_trans = ms[4];
+    times->v_hadamard = ms[5];
+    times->v_ifft = ms[6];
+    times->vlist = ms[3] + ms[4] + ms[5] + ms[6];
+    times->wlist = ms[7];
+    times->xlist = ms[8];
+    times->down = ms[9];
+    times->copy = ms[10];
+    times->total = tot;
+  }
+  return KIFMM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int kifmm_gpu_available(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n > 0 ? 1 : 0;
+}
+
+int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, const kifmm_gpu_options* opts,
+                     kifmm_gpu** out) {
+  if (!tree || !ops || !out) {
+    set_error("kifmm_gpu_create: null argument");
+    return KIFMM_EINVAL;
+  }
+  *out = nullptr;
+  if (tree->np != 3 && tree->np != 4 && tree->np != 6) {
+    set_error("kifmm_gpu_create: NP must be 3, 4 or 6 on the GPU (got " + std::to_string(tree->np) + ")");
+    return KIFMM_EUNSUP;
+  }
+  if (ops->np != tree->np || ops->n_e != tree->np * tree->np * tree->np - (tree->np - 2) * (tree->np - 2) * (tree->np - 2)) {
+    set_error("kifmm_gpu_create: operators do not match the tree's NP");
+    return KIFMM_EINVAL;
+  }
+  if (tree->n_nodes < 1 || tree->n_src < 0 || tree->n_trg < 0 || (tree->n_nodes - 1) % 8 != 0) {
+    set_error("kifmm_gpu_create: malformed tree view");
+    return KIFMM_EINVAL;
+  }
+  /* every index the kernels derive is checked on the host before any device call */
+  {
+    const int pr = kifmm_gpu_plan(tree, ops, nullptr);
+    if (pr != KIFMM_OK) return pr;
+  }
+  kifmm_gpu* g = new (std::nothrow) kifmm_gpu();
+  if (!g) {
+    set_error("host allocation failed");
+    return KIFMM_ENOMEM;
+  }
+  g->dev = opts ? opts->device : 0;
+  g->precision = opts ? opts->precision : KIFMM_PREC_FP64;
+  if (g->precision != KIFMM_PREC_FP64) {
+    set_error("kifmm_gpu_create: only the FP64 path is available in this build");
+    delete g;
+    return KIFMM_EUNSUP;
+  }
+  int r = KIFMM_OK;
+  if (hipSetDevice(g->dev) != hipSuccess) {
+    set_error("hipSetDevice failed");
+    delete g;
+    return KIFMM_EDEVICE;
+  }
+  if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("hipStreamCreate failed");
+    delete g;
+    return KIFMM_EDEVICE;
+  }
+  for (auto& e : g->ev) {
+    if (hipEventCreate(&e) != hipSuccess) {
+      set_error("hipEventCreate failed");
+      kifmm_gpu_destroy(g);
+      return KIFMM_EDEVICE;
+    }
+  }
+  try {
+    r = build(g, tree, ops);
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed in kifmm_gpu_create");
+    r = KIFMM_ENOMEM;
+  }
+  if (r != KIFMM_OK) {
+    kifmm_gpu_destroy(g);
+    return r;
+  }
+  if (hipDeviceSynchronize() != hipSuccess) {
+    set_error("device synchronisation failed after upload");
+    kifmm_gpu_destroy(g);
+    return KIFMM_EDEVICE;
+  }
+  *out = g;
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_evaluate(kifmm_gpu* g, double* pot_leaf, kifmm_phase_times* times) {
+  if (!g) {
+    set_error("kifmm_gpu_evaluate: null handle");
+    return KIFMM_EINVAL;
+  }
+  return evaluate(g, KIFMM_PHASE_ALL, pot_leaf, times);
+}
+
+int kifmm_gpu_evaluate_phases(kifmm_gpu* g, unsigned phases, double* pot_leaf, kifmm_phase_times* times) {
+  if (!g) {
+    set_error("kifmm_gpu_evaluate_phases: null handle");
+    return KIFMM_EINVAL;
+  }
+  return evaluate(g, phases & KIFMM_PHASE_ALL, pot_leaf, times);
+}
+
+int kifmm_gpu_destroy(kifmm_gpu* g) {
+  if (!g) return KIFMM_OK;
+  (void)hipSetDevice(g->dev);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  for (void* p : g->allocs) (void)hipFree(p);
+  for (auto& e : g->ev)
+    if (e) (void)hipEventDestroy(e);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifmm_work_counts* out) {
+  if (!tree || !ops) {
+    set_error("kifmm_gpu_plan: null argument");
+    return KIFMM_EINVAL;
+  }
+  if (tree->np != 3 && tree->np != 4 && tree->np != 6) {
+    set_error("kifmm_gpu_plan: NP must be 3, 4 or 6 on the GPU");
+    return KIFMM_EUNSUP;
+  }
+  Plan P;
+  int r = KIFMM_OK;
+  try {
+    r = make_plan(tree, ops, P);
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed in kifmm_gpu_plan");
+    return KIFMM_ENOMEM;
+  }
+  if (r == KIFMM_OK && out) *out = P.wc;
+  return r;
+}
+
+int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out) {
+  if (!g || !out) return KIFMM_EINVAL;
+  *out = g->counts;
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long count) {
+  if (!g || !out || count < 0) {
+    set_error("kifmm_gpu_copy_intermediate: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  const double* src = nullptr;
+  long avail = 0;
+  switch (which) {
+    case 0: src = g->uden; avail = (long)g->n_nodes * g->ld_e; break;
+    case 1: src = g->dchk; avail = (long)g->n_nodes * g->ld_e; break;
+    case 2: src = g->dden; avail = (long)g->n_nodes * g->ld_e; break;
+    case 3: src = g->pot; avail = g->n_trg; break;
+    default: set_error("kifmm_gpu_copy_intermediate: which must be 0..3"); return KIFMM_EINVAL;
+  }
+  if (count > avail) {
+    set_error("kifmm_gpu_copy_intermediate: count exceeds the array");
+    return KIFMM_EINVAL;
+  }
+  if (hipSetDevice(g->dev) != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess ||
+      hipMemcpy(out, src, sizeof(double) * count, hipMemcpyDeviceToHost) != hipSuccess) {
+    set_error("kifmm_gpu_copy_intermediate: copy failed");
+    return KIFMM_EDEVICE;
+  }
+  return KIFMM_OK;
+}
+
+}  // extern "C"

@@ -132,6 +132,11 @@ std::vector<int> sample_grid_offsets(int np) {
 
 /* ------------------------------------------------------------- the kernel */
 
+/* When set, every exact-kernel source sum runs in reverse order: a
+ * summation-order perturbation used to measure how much two correct FP64
+ * evaluations can differ (the conditioning floor of the pinv applies). */
+int g_reverse_order = 0;
+
 /* ulist__direct_evaluation (fmm/omp_sse_block.cc:266-327), pot += 1/4pi sum w/r */
 void direct_eval(int nt, const double* tx, const double* ty, const double* tz, double* pot,
                  int ns, const double* sx, const double* sy, const double* sz,
@@ -168,7 +173,8 @@ void direct_eval(int nt, const double* tx, const double* ty, const double* tz, d
   }
   for (; i < nt; i++) {
     double td = 0.0;
-    for (int j = 0; j < ns; j++) {
+    for (int jj = 0; jj < ns; jj++) {
+      const int j = g_reverse_order ? ns - 1 - jj : jj;
       double x = tx[i] - sx[j], y = ty[i] - sy[j], z = tz[i] - sz[j];
       double r2 = x * x + y * y + z * z;
       if (r2 != 0.0) td += w[j] / std::sqrt(r2);
@@ -1349,6 +1355,8 @@ int oracle_direct(int nt, const double* tx, const double* ty, const double* tz, 
   direct_eval(nt, tx, ty, tz, pot, ns, sx, sy, sz, w, mode);
   return 0;
 }
+
+void oracle_set_reverse_order(int on) { g_reverse_order = on ? 1 : 0; }
 
 int oracle_drand48(long seed, int k, double* out) {
   unsigned short st[3] = {0x330E, (unsigned short)(seed & 0xFFFF), (unsigned short)((seed >> 16) & 0xFFFF)};

@@ -90,6 +90,9 @@ int oracle_direct(int nt, const double* tx, const double* ty, const double* tz,
                   int ns, const double* sx, const double* sy, const double* sz,
                   const double* w, int kernel_mode, double* pot);
 int oracle_drand48(long seed, int k, double* out);
+/* Reverse every exact-kernel source sum (a summation-order perturbation
+ * for measuring the conditioning floor); process-wide, default off. */
+void oracle_set_reverse_order(int on);
 
 #ifdef __cplusplus
 }

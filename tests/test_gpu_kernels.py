@@ -1,0 +1,119 @@
+"""Kernel-level GPU tests: each evaluator kernel against numpy / the oracle.
+
+Pins the pieces the end-to-end parity test composes: the MFMA operand
+layout, the FP64 GEMM, the reciprocal square root, the forward/inverse
+pruned DFTs (FFTW r2c/c2r conventions: [z][y][x/2+1], unnormalised
+inverse, fmm/trans.cc:331-476) and the 316 translation spectra
+(fmm/omp_sse_block.cc:432-478).
+"""
+import numpy as np
+import pytest
+
+import kernel_lib as KL
+import kifmm_amd as K
+import oracle_lib as O
+
+pytestmark = pytest.mark.gpu
+
+
+def test_mfma_f64_operand_layout():
+    lanes = np.arange(64)
+    # A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i//4
+    rows = (lanes // 16)[:, None] + 4 * np.arange(4)[None, :]
+    cols = np.repeat((lanes % 16)[:, None], 4, axis=1)
+    out = KL.mfma_layout(lanes + 1.0, (lanes < 16).astype(float))   # D[i][j] = A[i][0] = i + 1
+    assert np.array_equal(out, rows + 1.0), f"row map differs:\n{out}"
+    out = KL.mfma_layout((lanes < 16).astype(float), lanes + 1.0)   # D[i][j] = B[0][j] = j + 1
+    assert np.array_equal(out, cols + 1.0), f"column map differs:\n{out}"
+
+
+@pytest.mark.parametrize("M,Kd,n", [(56, 152, 37), (152, 448, 20), (448, 56, 33), (26, 104, 5), (296, 1216, 17),
+                                    (152, 160, 64)])
+def test_gemm_matches_numpy(M, Kd, n):
+    rng = np.random.default_rng(M * 1000 + Kd)
+    A = rng.standard_normal((M, Kd))
+    B = rng.standard_normal((Kd, n))
+    se = rng.integers(-6, 7, n)
+    C0 = rng.standard_normal((M, n))
+    ref = (A @ B) * np.ldexp(1.0, se)[None, :]
+    got = KL.gemm(A, B, se)
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    got = KL.gemm(A, B, se, C0, accumulate=True)
+    assert np.max(np.abs(got - (C0 + ref))) <= 1e-13 * np.max(np.abs(C0 + ref))
+
+
+def test_rsqrt_accuracy():
+    rng = np.random.default_rng(3)
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-30), np.log(1e30), 200000)),
+                        rng.uniform(0.0, 12.0, 200000), [1.0, 4.0, 0.25, 2.0, 3.0]])
+    y, ym = KL.rsqrt(x)
+    exact = 1.0 / np.sqrt(x.astype(np.longdouble))
+    relerr = np.abs((y.astype(np.longdouble) - exact) / exact).astype(np.float64)
+    assert relerr.max() <= 2.0 ** -51, relerr.max()      # within 2 ulp
+    assert np.array_equal(y, ym)
+    y0, ym0 = KL.rsqrt(np.array([0.0, 1.0]))
+    assert ym0[0] == 0.0 and ym0[1] == 1.0
+
+
+def _grid_from_surface(np_, vals):
+    """scatter surface samples onto the (2np)^3 grid at offset np/2 (fmm/trans.cc:395-417)"""
+    N = 2 * np_
+    g = np.zeros((N, N, N))
+    cnt = 0
+    h = np_ // 2
+    for i in range(np_):
+        for j in range(np_):
+            for k in range(np_):
+                if i in (0, np_ - 1) or j in (0, np_ - 1) or k in (0, np_ - 1):
+                    g[k + h, j + h, i + h] = vals[cnt]
+                    cnt += 1
+    return g
+
+
+def _surface_from_grid(np_, g):
+    out = []
+    h = np_ // 2
+    for i in range(np_):
+        for j in range(np_):
+            for k in range(np_):
+                if i in (0, np_ - 1) or j in (0, np_ - 1) or k in (0, np_ - 1):
+                    out.append(g[k + h, j + h, i + h])
+    return np.array(out)
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6])
+def test_forward_dft_matches_numpy_rfftn(np_):
+    rng = np.random.default_rng(np_)
+    nb, ne = 3, K.n_e(np_)
+    den = rng.standard_normal((8 * nb, ne))
+    depth = rng.integers(2, 6, 8 * nb)
+    got = KL.dft_fwd(np_, den, depth)                   # [nb, F, 8]
+    for b in range(nb):
+        for o in range(8):
+            g = _grid_from_surface(np_, den[8 * b + o] * 2.0 ** depth[8 * b + o])
+            ref = np.fft.rfftn(g).reshape(-1)              # [kz][ky][kx] flattened
+            err = np.max(np.abs(got[b, :, o] - ref)) / np.max(np.abs(ref))
+            assert err <= 1e-14, (b, o, err)
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6])
+def test_inverse_dft_matches_numpy_irfftn(np_):
+    rng = np.random.default_rng(10 + np_)
+    N, nb = 2 * np_, 2
+    grids = rng.standard_normal((nb * 8, N, N, N))
+    spec = np.stack([np.fft.rfftn(grids[k]).reshape(-1) for k in range(nb * 8)])   # [nb*8, F]
+    spec = spec.reshape(nb, 8, -1).transpose(0, 2, 1).copy()                        # [nb, F, 8]
+    got = KL.dft_inv(np_, spec)
+    for k in range(nb * 8):
+        ref = _surface_from_grid(np_, grids[k])
+        assert np.max(np.abs(got[k] - ref)) <= 1e-13 * np.max(np.abs(ref)), k
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6])
+def test_translation_spectra_match_oracle(np_):
+    got = KL.translation_spectra(np_)                    # [F, 343]
+    ref = O.translation_spectra(np_, O.KERNEL_EXACT)     # [343, N, N, H]
+    ref = ref.reshape(343, -1).T
+    assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
+    near = [(a + 3) + 7 * (b + 3) + 49 * (c + 3) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
+    assert np.all(got[:, near] == 0)

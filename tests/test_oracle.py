@@ -1,0 +1,199 @@
+"""Pinning the CPU oracle (and the product's host primitives) to external facts.
+
+The reference ships no tests or golden vectors, so every check here comes
+from outside both implementations: glibc's drand48, closed forms, numpy's
+SVD and FFT, the dense kernel sum that the FFT-based M2L must reproduce,
+and brute force.
+"""
+import ctypes as C
+import itertools
+
+import numpy as np
+import pytest
+
+import kifmm_amd as K
+import oracle_lib as O
+
+OOFP = 1.0 / (4.0 * np.pi)
+
+
+def test_drand48_matches_glibc():
+    libc = C.CDLL("libc.so.6")
+    libc.drand48.restype = C.c_double
+    libc.srand48.argtypes = [C.c_long]
+    for seed in (0, 1, 2, 3, 26, 12345):
+        libc.srand48(seed)
+        ref = np.array([libc.drand48() for _ in range(1000)])
+        assert np.array_equal(O.drand48(seed, 1000), ref)
+        out = np.zeros(1000)
+        K.lib.kifmm_drand48(seed, 1000, out.ctypes.data_as(C.POINTER(C.c_double)))
+        assert np.array_equal(out, ref)
+
+
+@pytest.mark.parametrize("np_,R", [(3, 1.0), (4, 3.0), (6, 1.0), (8, 3.0)])
+def test_surface_closed_form(np_, R):
+    s = O.surface(np_, R)
+    n = np_ ** 3 - (np_ - 2) ** 3
+    assert s.shape == (3, n)
+    assert np.allclose(np.max(np.abs(s), axis=0), R, rtol=0, atol=1e-15)   # every point on the cube
+    step = 2.0 * R / (np_ - 1)
+    idx = np.rint((s + R) / step).astype(int)
+    assert np.allclose(idx * step - R, s, atol=1e-14)
+    keys = idx[0] * np_ * np_ + idx[1] * np_ + idx[2]
+    assert np.all(np.diff(keys) > 0)                                         # x slowest, z fastest
+    prod = np.zeros(3 * n)
+    K.lib.kifmm_surface(np_, R, prod.ctypes.data_as(C.POINTER(C.c_double)))
+    assert np.array_equal(prod.reshape(3, n), s)
+
+
+def test_regular_grid_wraps():
+    np_ = 4
+    g = O.regpos(np_)
+    N, step = 2 * np_, 2.0 / (np_ - 1)
+    i = np.arange(N)
+    w = np.where(i < np_, i, i - N) * step
+    X = np.broadcast_to(w[None, None, :], (N, N, N)).reshape(-1)
+    Z = np.broadcast_to(w[:, None, None], (N, N, N)).reshape(-1)
+    assert np.array_equal(g[0], X) and np.array_equal(g[2], Z)
+
+
+def kmat(src, trg):
+    d = src[:, :, None] - trg[:, None, :]
+    return OOFP / np.sqrt((d ** 2).sum(0))            # [ns, nt]: mat[i*tn + j]
+
+
+def np_pinv(M, eps):
+    U, s, Vt = np.linalg.svd(M, full_matrices=False)
+    r = int((s >= eps * s[0]).sum())
+    return (Vt[:r].T / s[:r]) @ U[:, :r].T
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6])
+def test_pinv_matches_numpy(np_):
+    eps = 0.1 ** (np_ + 1)
+    ue, uc = O.surface(np_, 1.0), O.surface(np_ + 2, 3.0)
+    de, dc = O.surface(np_, 3.0), O.surface(np_, 1.0)
+    tol = 1e-11 if np_ <= 4 else 1e-8
+    for src, trg in ((ue, uc), (de, dc)):
+        Km = kmat(src, trg).T                          # [n_check x n_equiv]
+        ref = np_pinv(Km, eps)
+        for got in (O.pinv(Km, eps), K_pinv(Km, eps)):
+            assert np.linalg.norm(got - ref) / np.linalg.norm(ref) < tol
+            # K P K = K up to the discarded singular values (< eps * sigma_0)
+            assert np.linalg.norm(Km @ got @ Km - Km) / np.linalg.norm(Km) < 20 * eps
+
+
+def K_pinv(M, eps):
+    M = np.asfortranarray(M)
+    m, n = M.shape
+    R = np.zeros((n, m), order="F")
+    K.lib.kifmm_pinv(M.ctypes.data_as(C.POINTER(C.c_double)), m, n, eps, R.ctypes.data_as(C.POINTER(C.c_double)))
+    return R
+
+
+def test_product_operators_match_definitions():
+    """UE2UC / DE2DC are plain kernel matrices: check against numpy from the
+    surfaces and the shifts of fmm/trans.cc:234-263 and 297-326."""
+    np_ = 4
+    p = K.Problem.create(1000, "uniform", 64, np_=np_, seed=1)
+    ops = p.operator_arrays()
+    ue, uc, de, dc = O.surface(np_, 1.0), O.surface(np_ + 2, 3.0), O.surface(np_, 3.0), O.surface(np_, 1.0)
+    for o in range(8):
+        idx = np.array([(o >> 2) & 1, (o >> 1) & 1, o & 1])[:, None]
+        ref = kmat(ue + (2 * idx - 1), 2.0 * uc).T
+        assert np.allclose(ops["ue2uc"][o], ref, rtol=1e-15, atol=0)
+        ref = kmat(de, 0.5 * dc + (idx - 0.5)).T
+        assert np.allclose(ops["de2dc"][o], ref, rtol=1e-15, atol=0)
+
+
+@pytest.mark.parametrize("N", [6, 8, 12])
+def test_fft_layout_matches_numpy(N):
+    rng = np.random.default_rng(N)
+    g = rng.standard_normal((N, N, N))
+    F = O.rfft3(g)
+    ref = np.fft.rfftn(g)
+    assert np.max(np.abs(F - ref)) < 1e-12 * np.max(np.abs(ref))
+    back = O.irfft3(ref)
+    assert np.max(np.abs(back - np.fft.irfftn(ref, s=(N, N, N), axes=(0, 1, 2)) * N ** 3)) < 1e-11 * N ** 3
+
+
+@pytest.mark.parametrize("np_", [3, 4, 6])
+@pytest.mark.parametrize("off", [(2, 0, 0), (-3, 2, 1), (3, 3, 3), (0, -2, 2)])
+def test_fft_m2l_equals_dense_kernel_sum(np_, off):
+    """The FFT route of the V-list (scatter, r2c, times the translation
+    spectrum, c2r / N^3, gather; fmm/omp_sse_block.cc:386-533) equals the
+    dense sum K(DC_target, UE_source) * den for a box at offset `off`."""
+    rng = np.random.default_rng(sum(off) + 7 * np_)
+    N, h = 2 * np_, np_ // 2
+    den = rng.standard_normal(O.n_e(np_))
+    ue, dc = O.surface(np_, 1.0), O.surface(np_, 1.0)
+    src = ue + 2.0 * np.array(off)[:, None]
+    dense = kmat(src, dc).T @ den
+    g = np.zeros((N, N, N))
+    pos = []
+    for i, j, k in itertools.product(range(np_), repeat=3):
+        if i in (0, np_ - 1) or j in (0, np_ - 1) or k in (0, np_ - 1):
+            pos.append((k + h, j + h, i + h))
+    for c, (a, b, d) in enumerate(pos):
+        g[a, b, d] = den[c]
+    T = O.translation_spectra(np_)
+    tid = (off[0] + 3) + 7 * (off[1] + 3) + 49 * (off[2] + 3)
+    y = O.irfft3(O.rfft3(g) * T[tid]) / N ** 3
+    fft = np.array([y[a, b, d] for (a, b, d) in pos])
+    assert np.max(np.abs(fft - dense)) < 1e-12 * np.max(np.abs(dense))
+
+
+def test_direct_known_answers():
+    one = [np.array([0.0]), np.array([0.0]), np.array([0.0])]
+    unit = [np.array([1.0]), np.array([0.0]), np.array([0.0])]
+    assert O.direct(one, unit, np.array([1.0]))[0] == pytest.approx(OOFP, rel=1e-16)
+    # r == 0 is excluded, superposition holds
+    pts = [np.array([0.0, 0.0, 0.3]), np.array([0.0, 0.5, 0.0]), np.array([0.0, 0.0, 0.0])]
+    p = O.direct([a[:1] for a in pts], pts, np.array([2.0, 3.0, 5.0]))[0]
+    assert p == pytest.approx(OOFP * (3.0 / 0.5 + 5.0 / 0.3), rel=1e-15)
+
+
+def test_oracle_sampled_error_config1():
+    """65,536 source = target points, NP = 4, q = 64 (the reference's CPU case)."""
+    o = O.Oracle(65536, "uniform", 64, 4, 0, 1)
+    o.build_operators(O.KERNEL_EXACT)
+    pot, _ = o.evaluate(O.KERNEL_EXACT)
+    err, _, _ = o.validate(pot, 1000)
+    assert 1e-6 < err < 1e-4
+    o2 = O.Oracle(65536, "uniform", 64, 4, 0, 1)
+    pot2, _ = o2.evaluate(O.KERNEL_REF)       # rsqrtps + one Newton step, operators built the same way
+    assert np.linalg.norm(pot2 - pot) / np.linalg.norm(pot) < 1e-5
+    err2, _, _ = o2.validate(pot2, 1000)
+    assert err2 < 1e-4
+
+
+def test_oracle_sampled_error_np6():
+    o = O.Oracle(32768, "uniform", 64, 6, 1, 0)
+    o.build_operators(O.KERNEL_EXACT)
+    pot, _ = o.evaluate(O.KERNEL_EXACT)
+    err, _, _ = o.validate(pot, 1000)
+    assert err < 1e-6
+
+
+@pytest.mark.parametrize("np_,case", [(4, (65536, "uniform", 64, 4, 0, 1)), (6, (32768, "uniform", 64, 6, 1, 0))])
+def test_summation_order_floor(np_, case):
+    """Two correct FP64 evaluations that differ only in summation order
+    differ by this much; the GPU parity bound per NP is set from it."""
+    n, dist, q, _, seed, c = case
+    p = K.Problem.create(n, dist, q, np_=np_, seed=seed, coincide=c)
+    raw = [np.ctypeslib.as_array(a, shape=(k,)).copy() for a, k in
+           ((p.operators().uc2ue, K.n_e(np_) * K.n_uc(np_)), (p.operators().ue2uc, 8 * K.n_uc(np_) * K.n_e(np_)),
+            (p.operators().dc2de, K.n_e(np_) ** 2), (p.operators().de2dc, 8 * K.n_e(np_) ** 2))]
+    pots = []
+    for rev in (0, 1):
+        O.set_reverse_order(rev)
+        try:
+            o = O.Oracle(n, dist, q, np_, seed, c)
+            o.set_operators(*raw)
+            pots.append(o.evaluate(O.KERNEL_EXACT)[0])
+        finally:
+            O.set_reverse_order(0)
+    d = np.linalg.norm(pots[0] - pots[1]) / np.linalg.norm(pots[0])
+    assert d <= O.PARITY_RTOL[np_] / 2
+    if np_ == 6:
+        assert d > 2e-12        # the floor is real: 1e-12 is below what FP64 order changes allow at NP = 6
