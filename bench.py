@@ -1,0 +1,162 @@
+#!/usr/bin/env python3
+"""Benchmark: Mpoints/s of one full FP64 KIFMM evaluation on one MI355X.
+
+Workload (default): 4,194,304 points i.i.d. uniform in [0,1]^3 (mapped to
+the reference root box by x' = 2x - 1), separate targets, densities in
+[0,1), seed 1, NP = 6, 512 points per box -- the plain FP64 GPU case the
+reference's fmmd--cuda path runs.  A step is one kifmm_gpu_evaluate over
+inputs already resident in HBM (up, U, V, W, X, down); the device -> host
+copy of the potentials is reported separately and is not part of `value`.
+
+Prints one JSON line.  Per-kernel HIP event times come from the library's
+own stream.  --gpus above 1 exits: this build runs on one GPU.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "kifmm--hybrid--double-only_amd"))
+
+P2P_FLOPS_PER_PAIR = 11.0  # the reference's convention, fmm/driver.cc:60
+FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and dense matrix)
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=4194304)
+    ap.add_argument("--dist", default="uniform")
+    ap.add_argument("--q", type=int, default=512)
+    ap.add_argument("--np", dest="np_", type=int, default=6)
+    ap.add_argument("--seed", type=int, default=1)
+    ap.add_argument("--nv", type=int, default=200, help="validation samples (0 = skip)")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    return ap.parse_args()
+
+
+def read_traffic():
+    """HBM bytes per U-list launch from the committed rocprofv3 counter
+    summary (FETCH_SIZE + WRITE_SIZE, calibrated on a copy kernel)."""
+    f = ROOT / "profiles" / "p2p_ulist_uniform4M_np6_counters.json"
+    if not f.exists():
+        return None
+    try:
+        return float(json.loads(f.read_text())["hbm_bytes_per_launch"])
+    except Exception:
+        return None
+
+
+def cpu_baseline(args, n_threads):
+    """The oracle (restated omp_sse_block, rsqrtps + Newton kernel, OpenMP)
+    on the same workload: evaluation timed with operators prebuilt."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle_lib as O
+
+    o = O.Oracle(args.n, args.dist, args.q, args.np_, args.seed, 0)
+    o.build_operators(O.KERNEL_REF)
+    _, t = o.evaluate(O.KERNEL_REF, n_threads)
+    t_eval = t["total"]
+    return {"value": args.n / t_eval / 1e6, "unit": "Mpoints/s", "cores": n_threads, "kind": "port",
+            "sample": f"one full evaluation of the same {args.n}-point input (operators prebuilt, "
+                      f"rsqrtps+Newton kernel, OpenMP over {n_threads} threads); "
+                      f"phases s: up {t['up']:.2f} U {t['ulist']:.2f} V {t['vlist']:.2f} "
+                      f"W {t['wlist']:.2f} X {t['xlist']:.2f} down {t['down']:.2f}"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
+              f"(the multi-GPU sharding is a design in DESIGN.md, not a code path).", file=sys.stderr)
+        return 2
+    import kifmm_amd as K
+
+    if not K.gpu_available():
+        print("bench.py: no HIP device visible", file=sys.stderr)
+        return 1
+    t0 = time.perf_counter()
+    prob = K.Problem.create(args.n, args.dist, args.q, np_=args.np_, seed=args.seed, coincide=0)
+    prob.operators()
+    t_setup = time.perf_counter() - t0
+    setup = prob.setup_times()
+    t0 = time.perf_counter()
+    ev = K.GpuEvaluator(prob)
+    t_create = time.perf_counter() - t0
+    wc = ev.work_counts()
+
+    for _ in range(args.warmup):
+        ev.evaluate(copy=False)
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        _, t = ev.evaluate(copy=False)
+        phases.append(t)
+    wall = time.perf_counter() - t0
+    ms_step = wall / args.steps * 1e3
+    value = args.n / (ms_step * 1e-3) / 1e6
+    avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+
+    # one step with the copy back, for the PCIe-inclusive figure and validation
+    pot_leaf, tc = ev.evaluate(copy=True)
+    pot = prob.scatter(pot_leaf)
+    err = None
+    if args.nv > 0:
+        err, _, _ = prob.validate(pot, args.nv)
+
+    p2p_ms = avg["p2p_kernel"]
+    achieved = P2P_FLOPS_PER_PAIR * wc["u_pairs"] / (p2p_ms * 1e-3) / 1e12
+    roofline = {"bound": "mfma", "pipe": "FP64 VALU (MI355X FP64 vector peak equals its dense FP64 MFMA peak)",
+                "kernel": "k_p2p_ulist", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
+                "algorithmic": f"{P2P_FLOPS_PER_PAIR:g} flop/pair x {wc['u_pairs']:.4g} U-list pairs per launch",
+                "kernel_ms": p2p_ms}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+        cpu = cpu_baseline(args, threads)
+
+    out = {
+        "metric": "Mpoints/s per FMM eval (4M uniform), FP64 GPU path",
+        "value": value,
+        "unit": "Mpoints/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded drand48 points and densities)",
+        "config": {"workload": f"{args.dist} {args.n} points, separate targets, seed {args.seed}, "
+                               f"NP={args.np_}, {args.q} pts/box, FP64",
+                   "n_points": args.n, "np": args.np_, "pts_per_box": args.q, "precision": "fp64"},
+        "phases_ms": {k: round(v, 4) for k, v in avg.items()},
+        "mpoints_s_incl_d2h": args.n / ((ms_step + tc["copy"]) * 1e-3) / 1e6,
+        "d2h_ms": tc["copy"],
+        "sampled_rel_err": err,
+        "setup_s": {"host_tree_lists": setup["tree_lists"], "generate": setup["generate"],
+                    "operators": setup["operators"], "upload": t_create, "total_host": t_setup},
+        "work": {k: wc[k] for k in ("u_pairs", "s2m_pairs", "l2t_pairs", "v_entries", "v_cmac", "gemm_flops")},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
