@@ -172,7 +172,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
       tile[tid] = v;
       __syncthreads();
       const int ns = min(256, ((total - base + 63) / 64) * 64);
-      eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
+      if (nt > 0) eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane); /* waves past the item only stage tiles */
       __syncthreads();
     }
   }
@@ -226,66 +226,128 @@ __global__ __launch_bounds__(256) void k_s2m(const int4* __restrict__ items, /* 
 /* out[:, j] = (acc ? out[:, j] : 0) + 2^sexp[j] * A[M x K] * in[:, j]
  * A column-major with lda rows (zero-padded to a multiple of 16) and K a
  * multiple of 4 (zero-padded columns).  Column j reads K doubles at
- * in + in_off[j] and writes M doubles at out + out_off[j].  One block of
- * four waves owns 16 columns and 4*RT row tiles (blockIdx.y picks the
- * group); wave w owns tiles w, w+4, ... of its group.
- * v_mfma_f64_16x16x4f64 operand layout (measured on gfx950 by
+ * in + in_off[j] and writes M doubles at out + out_off[j].
+ * A block of four waves owns GEMM_NC columns and up to 4*RT row tiles of
+ * 16 (blockIdx.y picks the row group).  K is walked in chunks of GEMM_KC:
+ * the A chunk [rows x KC] and the B chunk [KC x NC] are staged in LDS, then
+ * wave w runs its row tiles w, w+4, ... against the NC/16 column tiles with
+ * v_mfma_f64_16x16x4f64.  Operand layout (measured on gfx950 by
  * test_mfma_f64_operand_layout): A[i][k] in lane i + 16k, B[k][j] in lane
  * j + 16k, D[i][j] in lane j + 16(i%4), accumulator element i/4. */
-constexpr int GEMM_KC = 64;
+constexpr int GEMM_KC = 16;
+constexpr int GEMM_NC = 32;
 template <int RT>
 __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* __restrict__ A, int lda,
                                                    int ncols, const int* __restrict__ in_off,
                                                    const int* __restrict__ out_off,
                                                    const int* __restrict__ sexp,
                                                    const double* __restrict__ in,
-                                                   double* __restrict__ out, int accumulate) {
-  __shared__ double Bs[GEMM_KC][17];
+                                                   double* __restrict__ out, int accumulate,
+                                                   int ksplit, double* __restrict__ ws, int ldw) {
+  constexpr int CT = GEMM_NC / 16;
+  constexpr int ROWS = 64 * RT;                 /* rows of A staged per block */
+  __shared__ double As[GEMM_KC][ROWS + 1];
+  __shared__ double Bs[GEMM_KC][GEMM_NC + 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int col0 = blockIdx.x * 16;
+  const int col0 = blockIdx.x * GEMM_NC;
   const int nrt = (M + 15) / 16;
   const int rt0 = blockIdx.y * 4 * RT;
-  v4d acc[RT];
+  const int row0 = rt0 * 16;
+  const int nrows = min(ROWS, lda - row0);      /* rows of this group held in A (padded) */
+  v4d acc[RT][CT];
 #pragma unroll
-  for (int r = 0; r < RT; r++) acc[r] = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < K; k0 += GEMM_KC) {
-    for (int e = tid; e < GEMM_KC * 16; e += 256) {
+  for (int r = 0; r < RT; r++)
+#pragma unroll
+    for (int c = 0; c < CT; c++) acc[r][c] = (v4d){0.0, 0.0, 0.0, 0.0};
+  /* split-K: blockIdx.z owns columns [kbeg, kend) of A; partial sums then
+   * go to the workspace and k_gemm_reduce adds them in a fixed order */
+  const int kbeg = blockIdx.z * ksplit, kend = min(K, kbeg + ksplit);
+  for (int k0 = kbeg; k0 < kend; k0 += GEMM_KC) {
+    const int kc = min(GEMM_KC, kend - k0);
+    /* A chunk: kc columns of nrows contiguous doubles */
+    for (int e = tid; e < GEMM_KC * ROWS; e += 256) {
+      const int k = e / ROWS, i = e % ROWS;
+      As[k][i] = (k < kc && i < nrows) ? A[(size_t)(row0 + i) + (size_t)(k0 + k) * lda] : 0.0;
+    }
+    /* B chunk: NC columns of kc contiguous doubles */
+    for (int e = tid; e < GEMM_KC * GEMM_NC; e += 256) {
       const int c = e / GEMM_KC, k = e % GEMM_KC, col = col0 + c;
       double v = 0.0;
-      if (col < ncols && k0 + k < K) v = in[(size_t)in_off[col] + k0 + k];
+      if (col < ncols && k < kc) v = in[(size_t)in_off[col] + k0 + k];
       Bs[k][c] = v;
     }
     __syncthreads();
-    const int kend = min(GEMM_KC, K - k0);
-    for (int kk = 0; kk < kend; kk += 4) {
-      const double b = Bs[kk + (lane >> 4)][lane & 15];
+    for (int kk = 0; kk < kc; kk += 4) {
+      double b[CT];
+#pragma unroll
+      for (int c = 0; c < CT; c++) b[c] = Bs[kk + (lane >> 4)][c * 16 + (lane & 15)];
 #pragma unroll
       for (int r = 0; r < RT; r++) {
-        const int rt = rt0 + wave + 4 * r;
-        if (rt < nrt) {
-          const double a = A[(size_t)(rt * 16 + (lane & 15)) + (size_t)(k0 + kk + (lane >> 4)) * lda];
-          acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[r], 0, 0, 0);
+        const int rl = (wave + 4 * r) * 16; /* row offset inside the group */
+        if (rt0 + wave + 4 * r < nrt) {
+          const double a = As[kk + (lane >> 4)][rl + (lane & 15)];
+#pragma unroll
+          for (int c = 0; c < CT; c++) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[r][c], 0, 0, 0);
         }
       }
     }
     __syncthreads();
   }
-  const int col = col0 + (lane & 15);
-  if (col >= ncols) return;
-  const double s = ldexp(1.0, sexp[col]);
-  double* o = out + out_off[col];
+  if (ws) {
 #pragma unroll
-  for (int r = 0; r < RT; r++) {
-    const int rt = rt0 + wave + 4 * r;
-    if (rt >= nrt) continue;
+    for (int c = 0; c < CT; c++) {
+      const int col = col0 + c * 16 + (lane & 15);
+      if (col >= ncols) continue;
+      double* o = ws + ((size_t)blockIdx.z * ncols + col) * ldw;
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-      const int row = rt * 16 + (lane >> 4) + 4 * i;
-      if (row < M) {
-        const double v = s * acc[r][i];
-        o[row] = accumulate ? o[row] + v : v;
+      for (int r = 0; r < RT; r++) {
+        const int rt = rt0 + wave + 4 * r;
+        if (rt >= nrt) continue;
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+          const int row = rt * 16 + (lane >> 4) + 4 * i;
+          if (row < M) o[row] = acc[r][c][i];
+        }
       }
     }
+    return;
+  }
+#pragma unroll
+  for (int c = 0; c < CT; c++) {
+    const int col = col0 + c * 16 + (lane & 15);
+    if (col >= ncols) continue;
+    const double s = ldexp(1.0, sexp[col]);
+    double* o = out + out_off[col];
+#pragma unroll
+    for (int r = 0; r < RT; r++) {
+      const int rt = rt0 + wave + 4 * r;
+      if (rt >= nrt) continue;
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int row = rt * 16 + (lane >> 4) + 4 * i;
+        if (row < M) {
+          const double v = s * acc[r][c][i];
+          o[row] = accumulate ? o[row] + v : v;
+        }
+      }
+    }
+  }
+}
+
+/* Sums the nsplit partial products of a split-K GEMM in split order and
+ * applies scale / accumulate as k_gemm_cols does. */
+static __global__ __launch_bounds__(256) void k_gemm_reduce(int M, int ncols, int nsplit, const double* __restrict__ ws,
+                                                     int ldw, const int* __restrict__ out_off,
+                                                     const int* __restrict__ sexp, double* __restrict__ out,
+                                                     int accumulate) {
+  const size_t n = (size_t)ncols * M;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int col = (int)(e / M), row = (int)(e % M);
+    double v = 0.0;
+    for (int z = 0; z < nsplit; z++) v += ws[((size_t)z * ncols + col) * ldw + row];
+    double* o = out + out_off[col] + row;
+    const double t = ldexp(1.0, sexp[col]) * v;
+    *o = accumulate ? *o + t : t;
   }
 }
 
@@ -441,59 +503,82 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, d
   }
 }
 
-/* Spectral M2L for all 8 children of a parent at once.  Lane = parent,
- * block row = one frequency f (uniform, so translation entries are scalar
- * loads).  The V-list of child s of parent p is every child c of p's 26
- * same-depth neighbours o that is not adjacent to s (fmm/input.cc:314-336).
- * With child offset d = 2o + (c - s) in [-3,3]^3 the translation id is
+/* Spectral M2L for all 8 children of a parent at once.  The V-list of child
+ * s of parent p is every child c of p's 26 same-depth neighbours o that is
+ * not adjacent to s (fmm/input.cc:314-336).  With child offset
+ * d = 2o + (c - s) in [-3,3]^3 the translation id is
  * (dx+3) + 7(dy+3) + 49(dz+3) (fmm/evaluate--cuda.cu:1713-1723), i.e. a
- * per-neighbour base plus a per-(c,s) constant.  Pairs that are adjacent
+ * per-neighbour base plus a per-(c,s) constant.  Adjacent pairs
  * (max|d| < 2) hit the 27 near-field ids, whose spectra are exactly zero,
  * so every (c,s) pair is computed unconditionally and adds exactly 0 where
- * the reference has no V entry.  Missing neighbours point at the all-zero
- * X slot 0. */
-template <int NP>
-__global__ __launch_bounds__(64) void k_hadamard(int nparents, const int* __restrict__ nbr, /* 27 X slots per parent */
-                                                 const int* __restrict__ yslot,            /* Y slot per parent */
-                                                 const double2* __restrict__ T,
-                                                 const double2* __restrict__ X,
-                                                 double2* __restrict__ Y) {
-  using G = Grid<NP>;
-  constexpr int F = G::F;
-  const int p = blockIdx.x * 64 + threadIdx.x;
-  const int f = blockIdx.y;
-  const double2* __restrict__ Tf = T + (size_t)f * 343;
-  const int pp = min(p, nparents - 1);
-  double yr[8], yi[8];
-#pragma unroll
-  for (int s = 0; s < 8; s++) { yr[s] = 0.0; yi[s] = 0.0; }
+ * the reference has no V entry.  Missing neighbours use the all-zero X
+ * slot 0.
+ *
+ * Block = one group of up to 64 parents (lane = parent) at one frequency f
+ * (blockIdx.y; uniform, so translation entries are scalar loads).  The
+ * union of the group's neighbour slots is staged in LDS once (216 slots
+ * for an aligned 4x4x4 group of parents instead of 64 x 26 reads from L2);
+ * wave h computes target children s in [4h, 4h+4). */
+constexpr int HAD_CAP = 232; /* LDS slots per group */
+template <int NP, int HALF>
+__device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsigned char* __restrict__ loc,
+                                              const double2* __restrict__ Tf, double yr[4], double yi[4]) {
 #pragma unroll 1
   for (int o = 0; o < 27; o++) {
     if (o == 13) continue; /* the parent itself: its children are all adjacent */
     const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
     const double2* __restrict__ To = Tf + (2 * ox + 3) + 7 * (2 * oy + 3) + 49 * (2 * oz + 3);
-    const int slot = nbr[(size_t)pp * 27 + o];
-    const double2* xb = X + ((size_t)slot * F + f) * 8;
+    const double2* xs = Xs[loc[o]];
     double xr[8], xi[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) { const double2 v = xb[c]; xr[c] = v.x; xi[c] = v.y; }
+    for (int c = 0; c < 8; c++) { const double2 v = xs[c]; xr[c] = v.x; xi[c] = v.y; }
 #pragma unroll
     for (int c = 0; c < 8; c++) {
 #pragma unroll
-      for (int s = 0; s < 8; s++) {
-        const int ex = ((c >> 2) & 1) - ((s >> 2) & 1);
-        const int ey = ((c >> 1) & 1) - ((s >> 1) & 1);
-        const int ez = (c & 1) - (s & 1);
+      for (int q = 0; q < 4; q++) {
+        const int sidx = 4 * HALF + q;
+        const int ex = ((c >> 2) & 1) - ((sidx >> 2) & 1);
+        const int ey = ((c >> 1) & 1) - ((sidx >> 1) & 1);
+        const int ez = (c & 1) - (sidx & 1);
         const double2 t = To[ex + 7 * ey + 49 * ez];
-        yr[s] = __builtin_fma(t.x, xr[c], __builtin_fma(-t.y, xi[c], yr[s]));
-        yi[s] = __builtin_fma(t.x, xi[c], __builtin_fma(t.y, xr[c], yi[s]));
+        yr[q] = __builtin_fma(t.x, xr[c], __builtin_fma(-t.y, xi[c], yr[q]));
+        yi[q] = __builtin_fma(t.x, xi[c], __builtin_fma(t.y, xr[c], yi[q]));
       }
     }
   }
-  if (p < nparents) {
-    double2* yb = Y + ((size_t)yslot[p] * F + f) * 8;
+}
+
+template <int NP>
+__global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ groups, /* {first parent, count, union beg, union size} */
+                                                  const int* __restrict__ uslots,  /* X slots of each union */
+                                                  const unsigned char* __restrict__ loc, /* 27 union indices per parent */
+                                                  const int* __restrict__ yslot,   /* Y slot per parent */
+                                                  const double2* __restrict__ T,
+                                                  const double2* __restrict__ X,
+                                                  double2* __restrict__ Y) {
+  using G = Grid<NP>;
+  constexpr int F = G::F;
+  __shared__ double2 Xs[HAD_CAP][9];
+  const int4 g = groups[blockIdx.x];
+  const int f = blockIdx.y;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int half = __builtin_amdgcn_readfirstlane(tid >> 6);
+  for (int e = tid; e < g.w * 8; e += 128) {
+    const int u = e >> 3, c = e & 7;
+    Xs[u][c] = X[((size_t)uslots[g.z + u] * F + f) * 8 + c];
+  }
+  __syncthreads();
+  const int p = g.x + min(lane, g.y - 1);
+  const double2* __restrict__ Tf = T + (size_t)f * 343;
+  double yr[4] = {0.0, 0.0, 0.0, 0.0}, yi[4] = {0.0, 0.0, 0.0, 0.0};
+  if (half == 0)
+    hadamard_half<NP, 0>(Xs, loc + (size_t)p * 27, Tf, yr, yi);
+  else
+    hadamard_half<NP, 1>(Xs, loc + (size_t)p * 27, Tf, yr, yi);
+  if (lane < g.y) {
+    double2* yb = Y + ((size_t)yslot[p] * F + f) * 8 + 4 * half;
 #pragma unroll
-    for (int s = 0; s < 8; s++) yb[s] = make_double2(yr[s], yi[s]);
+    for (int q = 0; q < 4; q++) yb[q] = make_double2(yr[q], yi[q]);
   }
 }
 

@@ -78,11 +78,16 @@ struct kifmm_gpu {
   GemmList s2m_gemm;
   std::vector<LevelOps> lv;
   int* vblocks = nullptr; int* vslot = nullptr; int n_vblocks = 0;
-  int* hnbr = nullptr; int* hslot = nullptr; int n_hpar = 0;
+  int* hslot = nullptr; int n_hpar = 0;
+  int4* hgroups = nullptr; int n_hgroups = 0;
+  int* huslots = nullptr;
+  unsigned char* hloc = nullptr;
   int* wleaves = nullptr; int n_wleaves = 0;
   int* xnodes = nullptr; int n_xnodes = 0;
   int* l2t_leaves = nullptr; int n_l2t = 0;
   size_t chk_cols = 0;
+  double* ws = nullptr;
+  size_t ws_len = 0;
   kifmm_work_counts counts{};
 };
 
@@ -149,6 +154,28 @@ int find_box(const kifmm_tree_view* t, int d, const int want[3]) {
   return cur;
 }
 
+/* Launch shape of one column-list GEMM: enough blocks to fill the chip,
+ * splitting K (deterministically reduced) when the column count is small. */
+constexpr int GEMM_RT = 5;
+struct GemmShape {
+  int row_groups, nsplit, ksplit, ldw;
+};
+GemmShape gemm_shape(int ncols, int M, int K) {
+  GemmShape s;
+  const int nrt = (M + 15) / 16;
+  s.row_groups = (nrt + 4 * GEMM_RT - 1) / (4 * GEMM_RT);
+  const int blocks = ((ncols + GEMM_NC - 1) / GEMM_NC) * s.row_groups;
+  const int kchunks = (K + GEMM_KC - 1) / GEMM_KC;
+  int ns = 1;
+  if (blocks < 256) ns = std::min(std::min(16, kchunks), (512 + blocks - 1) / blocks);
+  ns = std::max(ns, 1);
+  const int per = (kchunks + ns - 1) / ns;           /* chunks per split */
+  s.ksplit = per * GEMM_KC;
+  s.nsplit = (K + s.ksplit - 1) / s.ksplit;
+  s.ldw = rup(M, 8);
+  return s;
+}
+
 /* Structural checks of a caller-supplied tree view: every index a kernel
  * derives from it must land inside the arrays it addresses. */
 int check_view(const kifmm_tree_view* t) {
@@ -204,7 +231,11 @@ struct Plan {
   HostGemm s2m_gemm;
   std::vector<HostLevel> lv;
   size_t chk_cols = 0;
+  size_t ws_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
+  std::vector<int4> hgroups;
+  std::vector<int> huslots;
+  std::vector<unsigned char> hloc;
   kifmm_work_counts wc{};
 };
 
@@ -288,7 +319,9 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     int col = 0;
     for (int i = t->level_start[d]; i < t->level_start[d + 1]; i++) {
       if (t->depth[i] != d) return fail("level_start does not match depth");
-      if (t->child[i] != -1 && (t->tag[i] & KIFMM_SRCNODE)) {
+      /* upward densities of depth 0 and 1 are never read: V-lists start at
+       * depth 2 and W-list boxes lie at depth >= 2 */
+      if (d >= 2 && t->child[i] != -1 && (t->tag[i] & KIFMM_SRCNODE)) {
         L.m2m_a.in.push_back(t->child[i] * lde); L.m2m_a.out.push_back(col * lduc); L.m2m_a.sexp.push_back(d + 1);
         L.m2m_b.in.push_back(col * lduc); L.m2m_b.out.push_back(i * lde); L.m2m_b.sexp.push_back(-d);
         col++;
@@ -304,6 +337,21 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       }
     }
     P.chk_cols = std::max(P.chk_cols, (size_t)col);
+  }
+  /* split-K workspace: the largest nsplit * ncols * ldw of any launch */
+  {
+    auto need = [&](size_t ncols, int M, int K) {
+      if (!ncols) return;
+      const GemmShape sh = gemm_shape((int)ncols, M, K);
+      if (sh.nsplit > 1) P.ws_len = std::max(P.ws_len, (size_t)sh.nsplit * ncols * sh.ldw);
+    };
+    need(P.s2m_gemm.in.size(), ne, lduc);
+    for (const HostLevel& L : P.lv) {
+      need(L.m2m_a.in.size(), nuc, 8 * lde);
+      need(L.m2m_b.in.size(), ne, lduc);
+      need(L.l2l.in.size(), 8 * lde, lde);
+      need(L.dc2de.in.size(), ne, lde);
+    }
   }
   /* GEMM columns: inputs read K doubles, outputs write M doubles */
   {
@@ -346,6 +394,35 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       return fail("V-list block out of range");
   for (int v : P.hnbr)
     if (v < 0 || v > P.nblk) return fail("V-list neighbour slot out of range");
+  /* Hadamard groups: runs of consecutive parents whose neighbour-slot union
+   * fits the LDS staging area; 64 parents (one lane each) when possible */
+  {
+    const int np_ = (int)P.hslot.size();
+    P.hloc.assign((size_t)np_ * 27, 0);
+    std::vector<int> uni;
+    int first = 0;
+    while (first < np_) {
+      int cnt = std::min(64, np_ - first);
+      for (;;) {
+        uni.assign(P.hnbr.begin() + (size_t)first * 27, P.hnbr.begin() + (size_t)(first + cnt) * 27);
+        std::sort(uni.begin(), uni.end());
+        uni.erase(std::unique(uni.begin(), uni.end()), uni.end());
+        if ((int)uni.size() <= HAD_CAP || cnt == 1) break;
+        cnt = std::max(1, cnt / 2);
+      }
+      if ((int)uni.size() > HAD_CAP) return fail("Hadamard neighbour union exceeds the LDS capacity");
+      P.hgroups.push_back(make_int4(first, cnt, (int)P.huslots.size(), (int)uni.size()));
+      for (int k = 0; k < cnt * 27; k++) {
+        const int slot = P.hnbr[(size_t)first * 27 + k];
+        P.hloc[(size_t)first * 27 + k] = (unsigned char)(std::lower_bound(uni.begin(), uni.end(), slot) - uni.begin());
+      }
+      P.huslots.insert(P.huslots.end(), uni.begin(), uni.end());
+      first += cnt;
+    }
+    for (const int4& gq : P.hgroups)
+      if (gq.y < 1 || gq.y > 64 || gq.x + gq.y > np_ || gq.w > HAD_CAP || gq.z + gq.w > (int)P.huslots.size())
+        return fail("Hadamard group out of range");
+  }
   wc.v_entries = t->v_ptr[nn];
   wc.v_cmac_ref = 0.5 * (2.0 * P.F) * wc.v_entries;
   wc.v_cmac = 26.0 * 64.0 * P.F * (double)P.hslot.size();
@@ -509,11 +586,16 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   }
   g->chk_cols = P.chk_cols;
   TRY(dev_zero(g, &g->chk, std::max<size_t>(P.chk_cols, 1) * lduc));
+  g->ws_len = P.ws_len;
+  TRY(dev_zero(g, &g->ws, std::max<size_t>(P.ws_len, 1)));
   g->n_vblocks = (int)P.vblocks.size();
   g->n_hpar = (int)P.hslot.size();
   TRY(dev_upload(g, &g->vblocks, P.vblocks.data(), P.vblocks.size()));
   TRY(dev_upload(g, &g->vslot, P.vslot.data(), P.vslot.size()));
-  TRY(dev_upload(g, &g->hnbr, P.hnbr.data(), P.hnbr.size()));
+  g->n_hgroups = (int)P.hgroups.size();
+  TRY(dev_upload(g, &g->hgroups, P.hgroups.data(), P.hgroups.size()));
+  TRY(dev_upload(g, &g->huslots, P.huslots.data(), P.huslots.size()));
+  TRY(dev_upload(g, &g->hloc, P.hloc.data(), P.hloc.size()));
   TRY(dev_upload(g, &g->hslot, P.hslot.data(), P.hslot.size()));
   const size_t spec = (size_t)(P.nblk + 1) * g->F * 8;
   TRY(dev_zero(g, &g->X, spec));
@@ -540,11 +622,24 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
 int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda, const double* in, double* out,
          int accumulate) {
   if (L.n == 0) return KIFMM_OK;
-  constexpr int RT = 4;
-  const int nrt = (M + 15) / 16;
-  dim3 grid((L.n + 15) / 16, (nrt + 4 * RT - 1) / (4 * RT));
-  hipLaunchKernelGGL(k_gemm_cols<RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
-                     L.sexp, in, out, accumulate);
+  const GemmShape sh = gemm_shape(L.n, M, K);
+  dim3 grid((L.n + GEMM_NC - 1) / GEMM_NC, sh.row_groups, sh.nsplit);
+  if (sh.nsplit == 1) {
+    hipLaunchKernelGGL(k_gemm_cols<GEMM_RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
+                       L.sexp, in, out, accumulate, K, (double*)nullptr, 0);
+    LAUNCH_CHECK();
+    return KIFMM_OK;
+  }
+  if ((size_t)sh.nsplit * L.n * sh.ldw > g->ws_len) {
+    set_error("internal: GEMM workspace too small");
+    return KIFMM_EINVAL;
+  }
+  hipLaunchKernelGGL(k_gemm_cols<GEMM_RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
+                     L.sexp, in, out, accumulate, sh.ksplit, g->ws, sh.ldw);
+  LAUNCH_CHECK();
+  const size_t n = (size_t)L.n * M;
+  hipLaunchKernelGGL(k_gemm_reduce, dim3((int)std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, g->stream, M,
+                     L.n, sh.nsplit, g->ws, sh.ldw, L.out_off, L.sexp, out, accumulate);
   LAUNCH_CHECK();
   return KIFMM_OK;
 }
@@ -566,9 +661,9 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   hipLaunchKernelGGL(k_trans<NP>, dim3(343), dim3(256), 0, g->stream, g->tw, g->T);
   LAUNCH_CHECK();
   HIP_TRY(hipEventRecord(e2, g->stream));
-  if (g->n_hpar > 0)
-    hipLaunchKernelGGL(k_hadamard<NP>, dim3((g->n_hpar + 63) / 64, g->F), dim3(64), 0, g->stream, g->n_hpar,
-                       g->hnbr, g->hslot, g->T, g->X, g->Y);
+  if (g->n_hgroups > 0)
+    hipLaunchKernelGGL(k_hadamard<NP>, dim3(g->n_hgroups, g->F), dim3(128), 0, g->stream, g->hgroups, g->huslots,
+                       g->hloc, g->hslot, g->T, g->X, g->Y);
   LAUNCH_CHECK();
   HIP_TRY(hipEventRecord(e3, g->stream));
   if (g->n_vblocks > 0)
