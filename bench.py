@@ -43,7 +43,26 @@ def parse():
     ap.add_argument("--nv", type=int, default=200, help="validation samples (0 = skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     return ap.parse_args()
+
+
+def calibrate_copy(mib):
+    """A copy kernel of known size: FETCH_SIZE / WRITE_SIZE read on it give
+    the scale from counter units to bytes for the traffic figures."""
+    import torch
+
+    n = mib * (1 << 20) // 8
+    a = torch.rand(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.neg(a, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_kernel": "torch.neg float64", "bytes_read": n * 8, "bytes_written": n * 8,
+                      "launches": 5}))
+    return 0
 
 
 def read_traffic():
@@ -81,6 +100,8 @@ def main():
         print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design in DESIGN.md, not a code path).", file=sys.stderr)
         return 2
+    if args.calibrate_copy:
+        return calibrate_copy(args.calibrate_copy)
     import kifmm_amd as K
 
     if not K.gpu_available():
