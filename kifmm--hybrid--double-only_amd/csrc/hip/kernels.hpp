@@ -116,69 +116,86 @@ __device__ __forceinline__ int stage_sources(double4* tile, int cap, const doubl
 
 /* ------------------------------------------------------------------ U-list */
 
-/* One block = 4 waves = up to 32 targets of one target leaf.  item = {leaf
- * range index, first target, target count}.  The U-list of the leaf is a
- * list of merged source ranges {beg, num}. */
+/* ------------------------------------------------------------------ U-list */
+
+/* U-list work item = {source range set, first target, target count, leaf},
+ * up to P2P_MAXT targets of one target leaf.  A block of 4 waves takes one
+ * item: target group g (8 targets) belongs to wave g % 4, so a wave carries
+ * up to P2P_G groups of accumulators and every staged source tile serves
+ * all the item's targets.  The sources are the leaf's U-list, a list of
+ * merged ranges {beg, num} of the gathered source array, walked in order.
+ * (The 8 targets of a group are wave-uniform and live in SGPRs: the inner
+ * loop is 3 sub + 3 for r^2, v_rsq_f64 + 5 refinement, 1 FMA per pair.) */
+constexpr int P2P_TILE = 512;
+constexpr int P2P_G = 4;
+constexpr int P2P_MAXT = 4 * 8 * P2P_G;
+
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items,
+__global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items, int n_items, int per_xcd,
                                                    const int* __restrict__ rng_ptr,
                                                    const int2* __restrict__ rng,
                                                    const double4* __restrict__ src4,
                                                    const double4* __restrict__ trg4,
                                                    double* __restrict__ pot) {
-  __shared__ double4 tile[256];
-  __shared__ int pre[65];
-  __shared__ int rbeg[64];
-  const int4 it = items[blockIdx.x];
+  __shared__ double4 tile[P2P_TILE];
+  /* workgroups go round-robin to the 8 XCDs: give each XCD a contiguous run
+   * of items (neighbouring leaves) so their shared sources stay in its L2 */
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const int4 it = items[item];
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int rfirst = rng_ptr[it.x], rlast = rng_ptr[it.x + 1];
-  /* the wave's 8 targets; slots past the item's count re-read the item's
-   * last target (always inside [it.y, it.y + it.z)) and are not stored */
-  const int t0 = it.y + 8 * wave;
-  const int nt = min(8, it.z - 8 * wave);
-  double tx[8], ty[8], tz[8], acc[8];
+  const int ngroups = (it.z + 7) >> 3;
+  double acc[P2P_G][8];
 #pragma unroll
-  for (int t = 0; t < 8; t++) {
-    const double4 q = trg4[it.y + min(8 * wave + t, it.z - 1)];
-    tx[t] = q.x; ty[t] = q.y; tz[t] = q.z; acc[t] = 0.0;
-  }
-  for (int r0 = rfirst; r0 < rlast; r0 += 64) {
-    const int nr = min(64, rlast - r0);
+  for (int k = 0; k < P2P_G; k++)
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[k][t] = 0.0;
+  int ri = rng_ptr[it.x], ro = 0;
+  const int rlast = rng_ptr[it.x + 1];
+  while (ri < rlast) {
+    /* fill the tile: block-uniform walk over the ranges (every range has
+     * num >= 1, so each pass consumes at least one source) */
     __syncthreads();
-    if (tid == 0) {
-      int s = 0;
-      for (int r = 0; r < nr; r++) {
-        const int2 g = rng[r0 + r];
-        pre[r] = s;
-        rbeg[r] = g.x;
-        s += g.y;
+    int fill = 0;
+    while (fill < P2P_TILE && ri < rlast) {
+      const int2 r = rng[ri];
+      const int take = min(r.y - ro, P2P_TILE - fill);
+      for (int k = tid; k < take; k += 256) tile[fill + k] = src4[r.x + ro + k];
+      fill += take;
+      ro += take;
+      if (ro == r.y) {
+        ri++;
+        ro = 0;
       }
-      pre[nr] = s;
     }
+    const int ns = (fill + 63) & ~63;
+    for (int k = fill + tid; k < ns; k += 256) tile[k] = make_double4(FAR, FAR, FAR, 0.0);
     __syncthreads();
-    const int total = pre[nr];
-    for (int base = 0; base < total; base += 256) {
-      const int p = base + tid;
-      double4 v = make_double4(FAR, FAR, FAR, 0.0);
-      if (p < total) {
-        int lo = 0, hi = nr - 1; /* last range with pre[r] <= p */
-        while (lo < hi) {
-          const int mid = (lo + hi + 1) >> 1;
-          if (pre[mid] <= p) lo = mid; else hi = mid - 1;
+#pragma unroll
+    for (int k = 0; k < P2P_G; k++) {
+      const int grp = wave + 4 * k;
+      if (grp < ngroups) {
+        /* slots past the count re-read the item's last target (in range) */
+        double tx[8], ty[8], tz[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++) {
+          const double4 q = trg4[it.y + min(8 * grp + t, it.z - 1)];
+          tx[t] = q.x; ty[t] = q.y; tz[t] = q.z;
         }
-        v = src4[rbeg[lo] + (p - pre[lo])];
+        eval8_lds<MASK>(tile, ns, tx, ty, tz, acc[k], lane);
       }
-      tile[tid] = v;
-      __syncthreads();
-      const int ns = min(256, ((total - base + 63) / 64) * 64);
-      if (nt > 0) eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane); /* waves past the item only stage tiles */
-      __syncthreads();
     }
   }
-  const double sum = wave_reduce8(acc, lane);
-  const int t = lane >> 3;
-  if ((lane & 7) == 0 && t < nt) pot[t0 + t] = OOFP_R * sum;
+#pragma unroll
+  for (int k = 0; k < P2P_G; k++) {
+    const int grp = wave + 4 * k;
+    if (grp < ngroups) {
+      const double sum = wave_reduce8(acc[k], lane);
+      const int t = 8 * grp + (lane >> 3);
+      if ((lane & 7) == 0 && t < it.z) pot[it.y + t] = OOFP_R * sum;
+    }
+  }
 }
 
 /* ------------------------------------------------------------------ S2M */

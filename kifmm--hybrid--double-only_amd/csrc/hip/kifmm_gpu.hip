@@ -272,7 +272,8 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     return KIFMM_EINVAL;
   };
 
-  /* U-list: merged source ranges per target leaf, items of <= 32 targets */
+  /* U-list: merged source ranges per target leaf; a leaf's targets split
+   * into the fewest items of <= P2P_MAXT, balanced, in multiples of 8 */
   P.rng_ptr.assign(1, 0);
   for (int i = 0; i < nn; i++) {
     if (t->child[i] != -1 || t->trg_num[i] <= 0) continue;
@@ -286,13 +287,16 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       else P.rng.push_back(make_int2(b, n));
     }
     P.rng_ptr.push_back((int)P.rng.size());
-    for (int c = 0; c < t->trg_num[i]; c += 32)
-      P.p2p_items.push_back(make_int4(set, t->trg_beg[i] + c, std::min(32, t->trg_num[i] - c), 0));
+    const int n = t->trg_num[i];
+    const int nit = (n + P2P_MAXT - 1) / P2P_MAXT;
+    const int per = (((n + nit - 1) / nit) + 7) & ~7;
+    for (int c = 0; c < n; c += per)
+      P.p2p_items.push_back(make_int4(set, t->trg_beg[i] + c, std::min(per, n - c), i));
   }
   /* k_p2p_ulist reads trg4[it.y .. it.y+it.z-1] and src4 inside each range */
   for (const int4& it : P.p2p_items)
-    if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > 32 || it.y < 0 ||
-        (long)it.y + it.z > t->n_trg)
+    if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > P2P_MAXT || it.y < 0 ||
+        (long)it.y + it.z > t->n_trg || it.w < 0 || it.w >= nn)
       return fail("U-list work item out of range");
   for (const int2& r : P.rng)
     if (r.x < 0 || r.y < 1 || (long)r.x + r.y > t->n_src) return fail("U-list source range out of range");
@@ -675,9 +679,11 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
 
 template <bool MASK>
 int point_phases_u(kifmm_gpu* g) {
-  if (g->n_p2p > 0)
-    hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(g->n_p2p), dim3(256), 0, g->stream, g->p2p_items, g->rng_ptr, g->rng,
-                       g->src4, g->trg4, g->pot);
+  if (g->n_p2p > 0) {
+    const int per_xcd = (g->n_p2p + 7) / 8;
+    hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items, g->n_p2p,
+                       per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
+  }
   LAUNCH_CHECK();
   return KIFMM_OK;
 }

@@ -29,6 +29,7 @@ CASES = [
     (32768, "ellipseUniformAngles", 16, 4, 2, 0),
     (20000, "gaussmix", 32, 4, 3, 0),
     (40000, "ellipsoid", 24, 6, 9, 1),
+    (60000, "uniform", 300, 4, 4, 0),         # leaves of 130-300 targets: several U-list items per leaf
 ]
 
 
