@@ -244,16 +244,17 @@ __global__ __launch_bounds__(256) void k_s2m(const int4* __restrict__ items, /* 
  * A column-major with lda rows (zero-padded to a multiple of 16) and K a
  * multiple of 4 (zero-padded columns).  Column j reads K doubles at
  * in + in_off[j] and writes M doubles at out + out_off[j].
- * A block of four waves owns GEMM_NC columns and up to 4*RT row tiles of
- * 16 (blockIdx.y picks the row group).  K is walked in chunks of GEMM_KC:
- * the A chunk [rows x KC] and the B chunk [KC x NC] are staged in LDS, then
- * wave w runs its row tiles w, w+4, ... against the NC/16 column tiles with
- * v_mfma_f64_16x16x4f64.  Operand layout (measured on gfx950 by
- * test_mfma_f64_operand_layout): A[i][k] in lane i + 16k, B[k][j] in lane
- * j + 16k, D[i][j] in lane j + 16(i%4), accumulator element i/4. */
+ * A block of four waves owns 16*RT rows (row group blockIdx.y) and 64*CT
+ * columns; wave w owns column tiles w*CT .. w*CT+CT-1 and runs all RT row
+ * tiles against them (RT*CT v_mfma_f64_16x16x4f64 accumulators).  K is
+ * walked in chunks of GEMM_KC staged in LDS; the next chunk is loaded into
+ * registers while the current one is multiplied.  Operand layout (measured
+ * on gfx950 by test_mfma_f64_operand_layout): A[i][k] in lane i + 16k,
+ * B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4.
+ * With ws != nullptr the unscaled partial product of K-slice blockIdx.z
+ * goes to ws[(z * ncols + j) * ldw + row] for k_gemm_reduce. */
 constexpr int GEMM_KC = 16;
-constexpr int GEMM_NC = 32;
-template <int RT>
+template <int RT, int CT>
 __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* __restrict__ A, int lda,
                                                    int ncols, const int* __restrict__ in_off,
                                                    const int* __restrict__ out_off,
@@ -261,90 +262,78 @@ __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* _
                                                    const double* __restrict__ in,
                                                    double* __restrict__ out, int accumulate,
                                                    int ksplit, double* __restrict__ ws, int ldw) {
-  constexpr int CT = GEMM_NC / 16;
-  constexpr int ROWS = 64 * RT;                 /* rows of A staged per block */
-  __shared__ double As[GEMM_KC][ROWS + 1];
-  __shared__ double Bs[GEMM_KC][GEMM_NC + 1];
+  constexpr int ROWS = 16 * RT, NC = 64 * CT;
+  /* row strides chosen so the 4 k-rows a wave reads hit disjoint bank halves */
+  constexpr int LA = ROWS + (((16 - ROWS) % 32) + 32) % 32;
+  constexpr int LB = NC + 16;
+  constexpr int AE = GEMM_KC * ROWS / 256, BE = GEMM_KC * NC / 256;
+  static_assert(GEMM_KC * ROWS % 256 == 0 && GEMM_KC * NC % 256 == 0, "chunk split");
+  __shared__ double As[GEMM_KC][LA];
+  __shared__ double Bs[GEMM_KC][LB];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int col0 = blockIdx.x * GEMM_NC;
-  const int nrt = (M + 15) / 16;
-  const int rt0 = blockIdx.y * 4 * RT;
-  const int row0 = rt0 * 16;
-  const int nrows = min(ROWS, lda - row0);      /* rows of this group held in A (padded) */
+  const int col0 = blockIdx.x * NC, row0 = blockIdx.y * ROWS;
+  const int kbeg = blockIdx.z * ksplit, kend = min(K, kbeg + ksplit);
   v4d acc[RT][CT];
 #pragma unroll
   for (int r = 0; r < RT; r++)
 #pragma unroll
     for (int c = 0; c < CT; c++) acc[r][c] = (v4d){0.0, 0.0, 0.0, 0.0};
-  /* split-K: blockIdx.z owns columns [kbeg, kend) of A; partial sums then
-   * go to the workspace and k_gemm_reduce adds them in a fixed order */
-  const int kbeg = blockIdx.z * ksplit, kend = min(K, kbeg + ksplit);
-  for (int k0 = kbeg; k0 < kend; k0 += GEMM_KC) {
-    const int kc = min(GEMM_KC, kend - k0);
-    /* A chunk: kc columns of nrows contiguous doubles */
-    for (int e = tid; e < GEMM_KC * ROWS; e += 256) {
-      const int k = e / ROWS, i = e % ROWS;
-      As[k][i] = (k < kc && i < nrows) ? A[(size_t)(row0 + i) + (size_t)(k0 + k) * lda] : 0.0;
+  double ra[AE], rb[BE];
+  auto load = [&](int k0) {
+#pragma unroll
+    for (int e = 0; e < AE; e++) {
+      const int idx = tid + 256 * e, k = idx / ROWS, i = idx % ROWS;
+      ra[e] = (k0 + k < kend && row0 + i < lda) ? A[(size_t)(row0 + i) + (size_t)(k0 + k) * lda] : 0.0;
     }
-    /* B chunk: NC columns of kc contiguous doubles */
-    for (int e = tid; e < GEMM_KC * GEMM_NC; e += 256) {
-      const int c = e / GEMM_KC, k = e % GEMM_KC, col = col0 + c;
-      double v = 0.0;
-      if (col < ncols && k < kc) v = in[(size_t)in_off[col] + k0 + k];
-      Bs[k][c] = v;
+#pragma unroll
+    for (int e = 0; e < BE; e++) {
+      const int idx = tid + 256 * e, c = idx / GEMM_KC, k = idx % GEMM_KC, col = col0 + c;
+      rb[e] = (col < ncols && k0 + k < kend) ? in[(size_t)in_off[col] + k0 + k] : 0.0;
+    }
+  };
+  load(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += GEMM_KC) {
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < AE; e++) {
+      const int idx = tid + 256 * e;
+      As[idx / ROWS][idx % ROWS] = ra[e];
+    }
+#pragma unroll
+    for (int e = 0; e < BE; e++) {
+      const int idx = tid + 256 * e;
+      Bs[idx % GEMM_KC][idx / GEMM_KC] = rb[e];
     }
     __syncthreads();
-    for (int kk = 0; kk < kc; kk += 4) {
+    if (k0 + GEMM_KC < kend) load(k0 + GEMM_KC);
+#pragma unroll
+    for (int kk = 0; kk < GEMM_KC; kk += 4) {
       double b[CT];
 #pragma unroll
-      for (int c = 0; c < CT; c++) b[c] = Bs[kk + (lane >> 4)][c * 16 + (lane & 15)];
+      for (int c = 0; c < CT; c++) b[c] = Bs[kk + (lane >> 4)][(wave * CT + c) * 16 + (lane & 15)];
 #pragma unroll
       for (int r = 0; r < RT; r++) {
-        const int rl = (wave + 4 * r) * 16; /* row offset inside the group */
-        if (rt0 + wave + 4 * r < nrt) {
-          const double a = As[kk + (lane >> 4)][rl + (lane & 15)];
+        const double a = As[kk + (lane >> 4)][r * 16 + (lane & 15)];
 #pragma unroll
-          for (int c = 0; c < CT; c++) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[r][c], 0, 0, 0);
-        }
+        for (int c = 0; c < CT; c++) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[c], acc[r][c], 0, 0, 0);
       }
     }
-    __syncthreads();
-  }
-  if (ws) {
-#pragma unroll
-    for (int c = 0; c < CT; c++) {
-      const int col = col0 + c * 16 + (lane & 15);
-      if (col >= ncols) continue;
-      double* o = ws + ((size_t)blockIdx.z * ncols + col) * ldw;
-#pragma unroll
-      for (int r = 0; r < RT; r++) {
-        const int rt = rt0 + wave + 4 * r;
-        if (rt >= nrt) continue;
-#pragma unroll
-        for (int i = 0; i < 4; i++) {
-          const int row = rt * 16 + (lane >> 4) + 4 * i;
-          if (row < M) o[row] = acc[r][c][i];
-        }
-      }
-    }
-    return;
   }
 #pragma unroll
   for (int c = 0; c < CT; c++) {
-    const int col = col0 + c * 16 + (lane & 15);
+    const int col = col0 + (wave * CT + c) * 16 + (lane & 15);
     if (col >= ncols) continue;
-    const double s = ldexp(1.0, sexp[col]);
-    double* o = out + out_off[col];
+    double* o = ws ? ws + ((size_t)blockIdx.z * ncols + col) * ldw : out + out_off[col];
+    const double s = ws ? 1.0 : ldexp(1.0, sexp[col]);
+    const bool add = !ws && accumulate;
 #pragma unroll
     for (int r = 0; r < RT; r++) {
-      const int rt = rt0 + wave + 4 * r;
-      if (rt >= nrt) continue;
 #pragma unroll
       for (int i = 0; i < 4; i++) {
-        const int row = rt * 16 + (lane >> 4) + 4 * i;
+        const int row = row0 + r * 16 + (lane >> 4) + 4 * i;
         if (row < M) {
           const double v = s * acc[r][c][i];
-          o[row] = accumulate ? o[row] + v : v;
+          o[row] = add ? o[row] + v : v;
         }
       }
     }
@@ -366,6 +355,67 @@ static __global__ __launch_bounds__(256) void k_gemm_reduce(int M, int ncols, in
     const double t = ldexp(1.0, sexp[col]) * v;
     *o = accumulate ? *o + t : t;
   }
+}
+
+/* Launch shape of one column-list GEMM (host side, shared by the evaluator
+ * and the kernel tests): the row-tile count picks the instantiation, and K
+ * is split (partials reduced in a fixed order) when there are too few
+ * blocks to fill the chip. */
+struct GemmShape {
+  int rt, ct, row_groups, col_blocks, nsplit, ksplit, ldw;
+};
+inline GemmShape gemm_shape(int ncols, int M, int K) {
+  GemmShape s;
+  const int tiles = (M + 15) / 16;
+  if (tiles <= 10) {
+    s.ct = 2;
+    s.rt = tiles <= 2 ? 2 : tiles <= 4 ? 4 : tiles <= 7 ? 7 : 10;
+    s.row_groups = 1;
+  } else {
+    s.ct = 1;
+    s.row_groups = (tiles + 19) / 20;
+    const int per = (tiles + s.row_groups - 1) / s.row_groups;
+    s.rt = per <= 14 ? 14 : 20;
+  }
+  s.col_blocks = (ncols + 64 * s.ct - 1) / (64 * s.ct);
+  const int blocks = s.col_blocks * s.row_groups;
+  const int kchunks = (K + GEMM_KC - 1) / GEMM_KC;
+  int ns = 1;
+  if (blocks < 256) ns = min(min(16, kchunks), (512 + blocks - 1) / blocks);
+  ns = max(ns, 1);
+  const int per = (kchunks + ns - 1) / ns;
+  s.ksplit = per * GEMM_KC;
+  s.nsplit = (K + s.ksplit - 1) / s.ksplit;
+  s.ldw = (M + 7) & ~7;
+  return s;
+}
+
+/* Launches k_gemm_cols (and k_gemm_reduce when K is split; ws must then
+ * hold nsplit * ncols * ldw doubles). */
+inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, int K, const double* A, int lda,
+                                   int ncols, const int* in_off, const int* out_off, const int* sexp,
+                                   const double* in, double* out, int accumulate, double* ws) {
+  const dim3 grid(sh.col_blocks, sh.row_groups, sh.nsplit);
+  double* w = sh.nsplit > 1 ? ws : nullptr;
+  const int ks = sh.nsplit > 1 ? sh.ksplit : K;
+#define KIFMM_GEMM_CASE(R, C)                                                                                  \
+  if (sh.rt == R && sh.ct == C)                                                                                \
+    hipLaunchKernelGGL((k_gemm_cols<R, C>), grid, dim3(256), 0, st, M, K, A, lda, ncols, in_off, out_off, sexp, \
+                       in, out, accumulate, ks, w, sh.ldw);
+  KIFMM_GEMM_CASE(2, 2)
+  else KIFMM_GEMM_CASE(4, 2)
+  else KIFMM_GEMM_CASE(7, 2)
+  else KIFMM_GEMM_CASE(10, 2)
+  else KIFMM_GEMM_CASE(14, 1)
+  else KIFMM_GEMM_CASE(20, 1)
+  else return hipErrorInvalidValue;
+#undef KIFMM_GEMM_CASE
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || sh.nsplit == 1) return e;
+  const size_t n = (size_t)ncols * M;
+  hipLaunchKernelGGL(k_gemm_reduce, dim3((int)min((size_t)2048, (n + 255) / 256)), dim3(256), 0, st, M, ncols,
+                     sh.nsplit, ws, sh.ldw, out_off, sexp, out, accumulate);
+  return hipGetLastError();
 }
 
 /* ---------------------------------------------------------- V-list DFTs */

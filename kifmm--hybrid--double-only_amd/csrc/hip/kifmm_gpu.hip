@@ -154,28 +154,6 @@ int find_box(const kifmm_tree_view* t, int d, const int want[3]) {
   return cur;
 }
 
-/* Launch shape of one column-list GEMM: enough blocks to fill the chip,
- * splitting K (deterministically reduced) when the column count is small. */
-constexpr int GEMM_RT = 5;
-struct GemmShape {
-  int row_groups, nsplit, ksplit, ldw;
-};
-GemmShape gemm_shape(int ncols, int M, int K) {
-  GemmShape s;
-  const int nrt = (M + 15) / 16;
-  s.row_groups = (nrt + 4 * GEMM_RT - 1) / (4 * GEMM_RT);
-  const int blocks = ((ncols + GEMM_NC - 1) / GEMM_NC) * s.row_groups;
-  const int kchunks = (K + GEMM_KC - 1) / GEMM_KC;
-  int ns = 1;
-  if (blocks < 256) ns = std::min(std::min(16, kchunks), (512 + blocks - 1) / blocks);
-  ns = std::max(ns, 1);
-  const int per = (kchunks + ns - 1) / ns;           /* chunks per split */
-  s.ksplit = per * GEMM_KC;
-  s.nsplit = (K + s.ksplit - 1) / s.ksplit;
-  s.ldw = rup(M, 8);
-  return s;
-}
-
 /* Structural checks of a caller-supplied tree view: every index a kernel
  * derives from it must land inside the arrays it addresses. */
 int check_view(const kifmm_tree_view* t) {
@@ -627,24 +605,16 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
          int accumulate) {
   if (L.n == 0) return KIFMM_OK;
   const GemmShape sh = gemm_shape(L.n, M, K);
-  dim3 grid((L.n + GEMM_NC - 1) / GEMM_NC, sh.row_groups, sh.nsplit);
-  if (sh.nsplit == 1) {
-    hipLaunchKernelGGL(k_gemm_cols<GEMM_RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
-                       L.sexp, in, out, accumulate, K, (double*)nullptr, 0);
-    LAUNCH_CHECK();
-    return KIFMM_OK;
-  }
-  if ((size_t)sh.nsplit * L.n * sh.ldw > g->ws_len) {
+  if (sh.nsplit > 1 && (size_t)sh.nsplit * L.n * sh.ldw > g->ws_len) {
     set_error("internal: GEMM workspace too small");
     return KIFMM_EINVAL;
   }
-  hipLaunchKernelGGL(k_gemm_cols<GEMM_RT>, grid, dim3(256), 0, g->stream, M, K, A, lda, L.n, L.in_off, L.out_off,
-                     L.sexp, in, out, accumulate, sh.ksplit, g->ws, sh.ldw);
-  LAUNCH_CHECK();
-  const size_t n = (size_t)L.n * M;
-  hipLaunchKernelGGL(k_gemm_reduce, dim3((int)std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, g->stream, M,
-                     L.n, sh.nsplit, g->ws, sh.ldw, L.out_off, L.sexp, out, accumulate);
-  LAUNCH_CHECK();
+  const hipError_t e = launch_gemm_cols(g->stream, sh, M, K, A, lda, L.n, L.in_off, L.out_off, L.sexp, in, out,
+                                        accumulate, g->ws);
+  if (e != hipSuccess) {
+    set_error(std::string("GEMM launch: ") + hipGetErrorString(e));
+    return KIFMM_EDEVICE;
+  }
   return KIFMM_OK;
 }
 

@@ -235,28 +235,16 @@ int kifmm_test_gemm(int M, int K, const double* A, int ncols, const double* Bm, 
     set_error("device allocation failed");
     return KIFMM_ENOMEM;
   }
-  /* same shape rule as the evaluator: split K when there are few columns */
-  constexpr int RT = 5;
-  const int nrt = (M + 15) / 16;
-  const int row_groups = (nrt + 4 * RT - 1) / (4 * RT);
-  const int blocks = ((ncols + GEMM_NC - 1) / GEMM_NC) * row_groups;
-  const int kchunks = (Kp + GEMM_KC - 1) / GEMM_KC;
-  int ns = blocks < 256 ? std::min(std::min(16, kchunks), (512 + blocks - 1) / blocks) : 1;
-  ns = std::max(ns, 1);
-  const int ksplit = ((kchunks + ns - 1) / ns) * GEMM_KC;
-  const int nsplit = (Kp + ksplit - 1) / ksplit;
-  const int ldw = rup(M, 8);
-  double* dws = nsplit > 1 ? B.up<double>(nullptr, (size_t)nsplit * ncols * ldw) : nullptr;
-  if (nsplit > 1 && !dws) {
+  /* the evaluator's shape rule and launcher */
+  const GemmShape sh = gemm_shape(ncols, M, Kp);
+  double* dws = sh.nsplit > 1 ? B.up<double>(nullptr, (size_t)sh.nsplit * ncols * sh.ldw) : nullptr;
+  if (sh.nsplit > 1 && !dws) {
     set_error("device allocation failed");
     return KIFMM_ENOMEM;
   }
-  hipLaunchKernelGGL(k_gemm_cols<RT>, dim3((ncols + GEMM_NC - 1) / GEMM_NC, row_groups, nsplit), dim3(256), 0, 0, M,
-                     Kp, dA, lda, ncols, din, dout, dse, dB, dC, accumulate, nsplit > 1 ? ksplit : Kp, dws, ldw);
-  if (nsplit > 1) {
-    const size_t n = (size_t)ncols * M;
-    hipLaunchKernelGGL(k_gemm_reduce, dim3((int)std::min<size_t>(2048, (n + 255) / 256)), dim3(256), 0, 0, M, ncols,
-                       nsplit, dws, ldw, dout, dse, dC, accumulate);
+  if (launch_gemm_cols(0, sh, M, Kp, dA, lda, ncols, din, dout, dse, dB, dC, accumulate, dws) != hipSuccess) {
+    set_error("GEMM launch failed");
+    return KIFMM_EDEVICE;
   }
   int r = finish("k_gemm_cols");
   if (r != KIFMM_OK) return r;

@@ -28,7 +28,8 @@ def test_mfma_f64_operand_layout():
 
 
 @pytest.mark.parametrize("M,Kd,n", [(56, 152, 37), (152, 448, 20), (448, 56, 33), (26, 104, 5), (296, 1216, 17),
-                                    (152, 160, 64), (152, 448, 9000), (56, 1216, 8200)])
+                                    (152, 160, 64), (152, 448, 9000), (56, 1216, 8200), (98, 56, 300),
+                                    (1216, 152, 700), (208, 26, 40)])
 def test_gemm_matches_numpy(M, Kd, n):
     """Small column counts take the split-K route (partials + ordered
     reduction), large ones the single-pass route."""
