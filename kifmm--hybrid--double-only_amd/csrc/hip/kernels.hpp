@@ -200,42 +200,50 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
 
 /* ------------------------------------------------------------------ S2M */
 
-/* One block per source leaf: UC check potential at c + r * SP_UC from the
- * leaf's sources (up_eval__gpu, fmm/cuda.cu:108-190 first half). */
+/* Lanes as targets, sources wave-uniform.  Every lane of the wave walks the
+ * same n sources (x, y, z, w), which the compiler reads with scalar loads
+ * and feeds to the VALU as SGPR operands: no LDS, no barrier, and no
+ * cross-lane reduction (each lane owns its sum).  Used where the sources of
+ * a whole wave are one short list: a leaf's points (S2M) or a box's
+ * equivalent surface (L2T).  Fixed summation order, deterministic. */
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
+__device__ __forceinline__ double sum_uniform(double tx, double ty, double tz, const double4* __restrict__ src,
+                                              int n) {
+  double acc0 = 0.0, acc1 = 0.0;
+  int k = 0;
+  for (; k + 2 <= n; k += 2) {
+    const double4 s0 = src[k], s1 = src[k + 1];
+    const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+    const double bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
+    acc0 = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+    acc1 = __builtin_fma(s1.w, inv_r<MASK>(__builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx))), acc1);
+  }
+  if (k < n) {
+    const double4 s0 = src[k];
+    const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+    acc0 = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+  }
+  return acc0 + acc1;
+}
+
+/* S2M check potentials (up_eval__gpu, fmm/cuda.cu:108-190 first half): one
+ * block per source leaf, lane j of the block is UC check point j at
+ * c + r * SP_UC[j]; the leaf's sources are wave-uniform.  Block size is
+ * n_uc rounded up to whole waves (at most 320 for NP = 6). */
+static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
                                              const double4* __restrict__ geo, /* node centre xyz + radius */
                                              const double* __restrict__ surf_uc, /* 3 x ld, unit surface */
                                              int n_uc, int ld_uc_surf,
                                              const double4* __restrict__ src4,
                                              double* __restrict__ chk, int ld_chk) {
-  __shared__ double4 tile[512];
   const int4 it = items[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int j = threadIdx.x, jj = min(j, n_uc - 1);
   const double4 g = geo[it.x];
-  double* out = chk + (size_t)it.w * ld_chk;
-  for (int base = 0; base < max(it.z, 1); base += 512) {
-    __syncthreads();
-    const int ns = stage_sources(tile, 512, src4, it.y + base, it.z - base, nullptr, 0, g, nullptr, tid, 256);
-    __syncthreads();
-    const int ngroups = (n_uc + 7) / 8;
-    for (int grp = wave; grp < ngroups; grp += 4) {
-      double tx[8], ty[8], tz[8], acc[8];
-#pragma unroll
-      for (int t = 0; t < 8; t++) {
-        const int j = min(grp * 8 + t, n_uc - 1);
-        tx[t] = g.x + g.w * surf_uc[j];
-        ty[t] = g.y + g.w * surf_uc[ld_uc_surf + j];
-        tz[t] = g.z + g.w * surf_uc[2 * ld_uc_surf + j];
-        acc[t] = 0.0;
-      }
-      eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
-      const double sum = wave_reduce8(acc, lane);
-      const int j = grp * 8 + (lane >> 3);
-      if ((lane & 7) == 0 && j < n_uc) out[j] = (base == 0 ? 0.0 : out[j]) + OOFP_R * sum;
-    }
-  }
+  const double tx = g.x + g.w * surf_uc[jj];
+  const double ty = g.y + g.w * surf_uc[ld_uc_surf + jj];
+  const double tz = g.z + g.w * surf_uc[2 * ld_uc_surf + jj];
+  const double acc = sum_uniform<false>(tx, ty, tz, src4 + it.y, it.z);
+  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * acc;
 }
 
 /* ------------------------------------------------------------ GEMM (MFMA) */
@@ -808,17 +816,35 @@ __global__ __launch_bounds__(256) void k_xlist(const int* __restrict__ nodes, co
 }
 
 /* L2T (down_eval__gpu_leaves, fmm/cuda.cu:1804-1872): one block per target leaf. */
-static __global__ __launch_bounds__(256) void k_l2t(const int* __restrict__ leaves, const int* __restrict__ tbeg,
-                                             const int* __restrict__ tnum, const double4* __restrict__ geo,
-                                             const double* __restrict__ surf_de, int ld_surf, int n_e,
-                                             const double* __restrict__ dden, int ld_e,
-                                             const double4* __restrict__ trg4, double* __restrict__ pot) {
-  __shared__ double4 tile[320];
-  const int t = leaves[blockIdx.x];
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int ns = stage_sources(tile, 320, nullptr, 0, n_e, surf_de, ld_surf, geo[t], dden + (size_t)t * ld_e, tid, 256);
-  __syncthreads();
-  apply_tile<false>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+/* L2T (down_eval__gpu_leaves, fmm/cuda.cu:1804-1872): item = {leaf, first
+ * target, count <= 64}, one wave; lane = target, the leaf's DE surface
+ * points c + r * SP_DE[j] with densities dden[leaf][j] are wave-uniform. */
+static __global__ __launch_bounds__(64) void k_l2t(const int4* __restrict__ items,
+                                                   const double4* __restrict__ geo,
+                                                   const double* __restrict__ surf_de, int ld_surf, int n_e,
+                                                   const double* __restrict__ dden, int ld_e,
+                                                   const double4* __restrict__ trg4, double* __restrict__ pot) {
+  const int4 it = items[blockIdx.x];
+  const int k = threadIdx.x;
+  const double4 q = trg4[it.y + min(k, it.z - 1)];
+  const double4 g = geo[it.x];
+  const double* __restrict__ den = dden + (size_t)it.x * ld_e;
+  double acc0 = 0.0, acc1 = 0.0;
+  int j = 0;
+  for (; j + 2 <= n_e; j += 2) {
+    const double ax = q.x - (g.x + g.w * surf_de[j]), ay = q.y - (g.y + g.w * surf_de[ld_surf + j]),
+                 az = q.z - (g.z + g.w * surf_de[2 * ld_surf + j]);
+    const double bx = q.x - (g.x + g.w * surf_de[j + 1]), by = q.y - (g.y + g.w * surf_de[ld_surf + j + 1]),
+                 bz = q.z - (g.z + g.w * surf_de[2 * ld_surf + j + 1]);
+    acc0 = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+    acc1 = __builtin_fma(den[j + 1], inv_r<false>(__builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx))), acc1);
+  }
+  if (j < n_e) {
+    const double ax = q.x - (g.x + g.w * surf_de[j]), ay = q.y - (g.y + g.w * surf_de[ld_surf + j]),
+                 az = q.z - (g.z + g.w * surf_de[2 * ld_surf + j]);
+    acc0 = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+  }
+  if (k < it.z) pot[it.y + k] += OOFP_R * (acc0 + acc1);
 }
 
 static __global__ void k_fill(double* __restrict__ p, size_t n, double v) {

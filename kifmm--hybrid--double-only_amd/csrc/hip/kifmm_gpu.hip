@@ -84,7 +84,7 @@ struct kifmm_gpu {
   unsigned char* hloc = nullptr;
   int* wleaves = nullptr; int n_wleaves = 0;
   int* xnodes = nullptr; int n_xnodes = 0;
-  int* l2t_leaves = nullptr; int n_l2t = 0;
+  int4* l2t_items = nullptr; int n_l2t = 0;
   size_t chk_cols = 0;
   double* ws = nullptr;
   size_t ws_len = 0;
@@ -211,6 +211,7 @@ struct Plan {
   size_t chk_cols = 0;
   size_t ws_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
+  std::vector<int4> l2t_items;
   std::vector<int4> hgroups;
   std::vector<int> huslots;
   std::vector<unsigned char> hloc;
@@ -432,6 +433,8 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     if (trg && leaf && t->trg_num[i] > 0 && t->depth[i] >= 2) {
       P.l2t.push_back(i);
       wc.l2t_pairs += (double)ne * t->trg_num[i];
+      for (int c = 0; c < t->trg_num[i]; c += 64)
+        P.l2t_items.push_back(make_int4(i, t->trg_beg[i] + c, std::min(64, t->trg_num[i] - c), 0));
     }
   }
   /* apply_tile clamps target indices to [tbeg, tbeg + tnum - 1]: the leaves
@@ -442,7 +445,11 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     if (t->child[i] == -1 && t->trg_num[i] < ne && t->trg_num[i] < 1) return fail("X-list leaf without targets");
   for (int i : P.l2t)
     if (t->trg_num[i] < 1) return fail("L2T leaf without targets");
-  if (ne > 320) return fail("NP too large for the L2T tile");
+  /* k_l2t reads trg4[it.y .. it.y + it.z - 1] and the leaf's DE densities */
+  for (const int4& it : P.l2t_items)
+    if (it.x < 0 || it.x >= nn || it.z < 1 || it.z > 64 || it.y < 0 || (long)it.y + it.z > t->n_trg)
+      return fail("L2T work item out of range");
+  if (nuc > 320) return fail("NP too large for the S2M block");
   return KIFMM_OK;
 }
 
@@ -585,10 +592,10 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
   g->n_wleaves = (int)P.wleaves.size();
   g->n_xnodes = (int)P.xnodes.size();
-  g->n_l2t = (int)P.l2t.size();
+  g->n_l2t = (int)P.l2t_items.size();
   TRY(dev_upload(g, &g->wleaves, P.wleaves.data(), P.wleaves.size()));
   TRY(dev_upload(g, &g->xnodes, P.xnodes.data(), P.xnodes.size()));
-  TRY(dev_upload(g, &g->l2t_leaves, P.l2t.data(), P.l2t.size()));
+  TRY(dev_upload(g, &g->l2t_items, P.l2t_items.data(), P.l2t_items.size()));
   return KIFMM_OK;
 }
 
@@ -686,12 +693,9 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
   TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
-    if (g->coincident)
-      hipLaunchKernelGGL(k_s2m<false>, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->surf_uc,
-                         g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
-    else
-      hipLaunchKernelGGL(k_s2m<false>, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->surf_uc,
-                         g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    /* sources never sit on the check surface: no r = 0 mask */
+    hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->geo,
+                       g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
     LAUNCH_CHECK();
     TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
@@ -729,8 +733,8 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
-    hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(256), 0, g->stream, g->l2t_leaves, g->tbeg, g->tnum, g->geo,
-                       g->surf_de, g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
+    hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de, g->ld_surf,
+                       g->n_e, g->dden, lde, g->trg4, g->pot);
   LAUNCH_CHECK();
   HIP_TRY(hipEventRecord(ev[9], g->stream));
   if (pot_leaf)
