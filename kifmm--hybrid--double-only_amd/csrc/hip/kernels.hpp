@@ -596,14 +596,14 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, d
  * wave h computes target children s in [4h, 4h+4). */
 constexpr int HAD_CAP = 232; /* LDS slots per group */
 template <int NP, int HALF>
-__device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsigned char* __restrict__ loc,
+__device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsigned char (*sloc)[64], int lane,
                                               const double2* __restrict__ Tf, double yr[4], double yi[4]) {
 #pragma unroll 1
   for (int o = 0; o < 27; o++) {
     if (o == 13) continue; /* the parent itself: its children are all adjacent */
     const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
     const double2* __restrict__ To = Tf + (2 * ox + 3) + 7 * (2 * oy + 3) + 49 * (2 * oz + 3);
-    const double2* xs = Xs[loc[o]];
+    const double2* xs = Xs[sloc[o][lane]];
     double xr[8], xi[8];
 #pragma unroll
     for (int c = 0; c < 8; c++) { const double2 v = xs[c]; xr[c] = v.x; xi[c] = v.y; }
@@ -623,6 +623,12 @@ __device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsi
   }
 }
 
+/* One block = one parent group (up to 64 parents, a lane each) and
+ * HAD_FB consecutive frequencies.  The union of the group's neighbour
+ * slots is staged in LDS per frequency; the next frequency's spectra are
+ * loaded into registers while the current one is multiplied, and the
+ * parents' 27 union indices are staged once per block. */
+constexpr int HAD_FB = 8;
 template <int NP>
 __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ groups, /* {first parent, count, union beg, union size} */
                                                   const int* __restrict__ uslots,  /* X slots of each union */
@@ -633,27 +639,59 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
                                                   double2* __restrict__ Y) {
   using G = Grid<NP>;
   constexpr int F = G::F;
+  static_assert(F % HAD_FB == 0, "frequency blocks");
+  constexpr int PER = (HAD_CAP * 8 + 127) / 128;
   __shared__ double2 Xs[HAD_CAP][9];
+  __shared__ unsigned char sloc[27][64];
   const int4 g = groups[blockIdx.x];
-  const int f = blockIdx.y;
+  const int f0 = blockIdx.y * HAD_FB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int half = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int e = tid; e < g.w * 8; e += 128) {
-    const int u = e >> 3, c = e & 7;
-    Xs[u][c] = X[((size_t)uslots[g.z + u] * F + f) * 8 + c];
-  }
-  __syncthreads();
-  const int p = g.x + min(lane, g.y - 1);
-  const double2* __restrict__ Tf = T + (size_t)f * 343;
-  double yr[4] = {0.0, 0.0, 0.0, 0.0}, yi[4] = {0.0, 0.0, 0.0, 0.0};
-  if (half == 0)
-    hadamard_half<NP, 0>(Xs, loc + (size_t)p * 27, Tf, yr, yi);
-  else
-    hadamard_half<NP, 1>(Xs, loc + (size_t)p * 27, Tf, yr, yi);
-  if (lane < g.y) {
-    double2* yb = Y + ((size_t)yslot[p] * F + f) * 8 + 4 * half;
+  const int ne8 = g.w * 8;
+  int us[PER];
 #pragma unroll
-    for (int q = 0; q < 4; q++) yb[q] = make_double2(yr[q], yi[q]);
+  for (int k = 0; k < PER; k++) {
+    const int e = tid + 128 * k;
+    us[k] = e < ne8 ? uslots[g.z + (e >> 3)] : 0;
+  }
+  double2 pre[PER];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = tid + 128 * k;
+      pre[k] = e < ne8 ? X[((size_t)us[k] * F + f) * 8 + (e & 7)] : make_double2(0.0, 0.0);
+    }
+  };
+  fetch(f0);
+  /* union indices of the block's parents, [neighbour][lane]; lanes past the
+   * group's count repeat its last parent */
+  for (int e = tid; e < 27 * 64; e += 128) {
+    const int l = e / 27, o = e - 27 * l;
+    sloc[o][l] = loc[(size_t)(g.x + min(l, g.y - 1)) * 27 + o];
+  }
+  const int p = g.x + min(lane, g.y - 1);
+  const int ys = yslot[p];
+  for (int fi = 0; fi < HAD_FB; fi++) {
+    const int f = f0 + fi;
+    __syncthreads(); /* the previous frequency's readers are done with Xs */
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = tid + 128 * k;
+      if (e < ne8) Xs[e >> 3][e & 7] = pre[k];
+    }
+    __syncthreads();
+    if (fi + 1 < HAD_FB) fetch(f + 1);
+    const double2* __restrict__ Tf = T + (size_t)f * 343;
+    double yr[4] = {0.0, 0.0, 0.0, 0.0}, yi[4] = {0.0, 0.0, 0.0, 0.0};
+    if (half == 0)
+      hadamard_half<NP, 0>(Xs, sloc, lane, Tf, yr, yi);
+    else
+      hadamard_half<NP, 1>(Xs, sloc, lane, Tf, yr, yi);
+    if (lane < g.y) {
+      double2* yb = Y + ((size_t)ys * F + f) * 8 + 4 * half;
+#pragma unroll
+      for (int q = 0; q < 4; q++) yb[q] = make_double2(yr[q], yi[q]);
+    }
   }
 }
 

@@ -643,7 +643,7 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   LAUNCH_CHECK();
   HIP_TRY(hipEventRecord(e2, g->stream));
   if (g->n_hgroups > 0)
-    hipLaunchKernelGGL(k_hadamard<NP>, dim3(g->n_hgroups, g->F), dim3(128), 0, g->stream, g->hgroups, g->huslots,
+    hipLaunchKernelGGL(k_hadamard<NP>, dim3(g->n_hgroups, g->F / HAD_FB), dim3(128), 0, g->stream, g->hgroups, g->huslots,
                        g->hloc, g->hslot, g->T, g->X, g->Y);
   LAUNCH_CHECK();
   HIP_TRY(hipEventRecord(e3, g->stream));
