@@ -1,5 +1,5 @@
 #!/usr/bin/env python3
-"""Benchmark: Mpoints/s of one full FP64 KIFMM evaluation on one MI355X.
+"""Benchmark: Mpoints/s of one full KIFMM evaluation on one MI355X.
 
 Workload (default): 4,194,304 points i.i.d. uniform in [0,1]^3 (mapped to
 the reference root box by x' = 2x - 1), separate targets, densities in
@@ -7,6 +7,10 @@ the reference root box by x' = 2x - 1), separate targets, densities in
 reference's fmmd--cuda path runs.  A step is one kifmm_gpu_evaluate over
 inputs already resident in HBM (up, U, V, W, X, down); the device -> host
 copy of the potentials is reported separately and is not part of `value`.
+
+`mixed_vs_fp64` adds the mixed-precision path against the FP64 path on
+4,194,304 uniform points at NP = 4, 256 points per box (same tree), each
+with its sampled error against the FP64 direct sum.
 
 Prints one JSON line.  Per-kernel HIP event times come from the library's
 own stream.  --gpus above 1 exits: this build runs on one GPU.
@@ -41,6 +45,9 @@ def parse():
     ap.add_argument("--np", dest="np_", type=int, default=6)
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--nv", type=int, default=200, help="validation samples (0 = skip)")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
+    ap.add_argument("--no-mixed-compare", action="store_true",
+                    help="skip the mixed vs FP64 comparison on 4M uniform points at NP = 4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -94,6 +101,43 @@ def cpu_baseline(args, n_threads):
                       f"W {t['wlist']:.2f} X {t['xlist']:.2f} down {t['down']:.2f}"}
 
 
+def timed(K, prob, precision, steps, warmup, graph=False):
+    """Mean wall ms per evaluation (inputs resident, result left on the
+    device) and mean HIP-event phase times."""
+    ev = K.GpuEvaluator(prob, precision=precision, graph=graph)
+    for _ in range(warmup):
+        ev.evaluate(copy=False)
+    phases = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        phases.append(ev.evaluate(copy=False)[1])
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
+    return ev, ms, avg
+
+
+def mixed_vs_fp64(K, steps, warmup, nv):
+    """4,194,304 uniform points, seed 1, NP = 4, 256 points per box: the
+    plain FP64 path and the mixed path on the same tree, Mpoints/s and
+    sampled error of each."""
+    n = 4194304
+    prob = K.Problem.create(n, "uniform", 256, np_=4, seed=1, coincide=0)
+    prob.operators()
+    out = {"workload": f"uniform {n} points, separate targets, seed 1, NP=4, 256 pts/box"}
+    for name, prec, graph in (("fp64", K.PREC_FP64, False), ("fp64_graph", K.PREC_FP64, True),
+                              ("mixed", K.PREC_MIXED, False), ("mixed_graph", K.PREC_MIXED, True)):
+        ev, ms, avg = timed(K, prob, prec, steps, warmup, graph)
+        err = prob.validate(prob.scatter(ev.evaluate(copy=True)[0]), nv)[0] if nv > 0 else None
+        out[name] = {"mpoints_s": n / (ms * 1e-3) / 1e6, "ms_per_step": ms, "sampled_rel_err": err}
+        if not graph:
+            out[name]["phases_ms"] = {k: round(v, 4) for k, v in avg.items()}
+        del ev
+    # the mixed, batched (one HIP graph) path against the plain FP64 path
+    out["ratio"] = out["mixed_graph"]["mpoints_s"] / out["fp64"]["mpoints_s"]
+    out["ratio_both_graphed"] = out["mixed_graph"]["mpoints_s"] / out["fp64_graph"]["mpoints_s"]
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -112,22 +156,14 @@ def main():
     prob.operators()
     t_setup = time.perf_counter() - t0
     setup = prob.setup_times()
+    prec = K.PREC_MIXED if args.precision == "mixed" else K.PREC_FP64
     t0 = time.perf_counter()
-    ev = K.GpuEvaluator(prob)
+    ev = K.GpuEvaluator(prob, precision=prec)
     t_create = time.perf_counter() - t0
     wc = ev.work_counts()
-
-    for _ in range(args.warmup):
-        ev.evaluate(copy=False)
-    phases = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        _, t = ev.evaluate(copy=False)
-        phases.append(t)
-    wall = time.perf_counter() - t0
-    ms_step = wall / args.steps * 1e3
+    del ev
+    ev, ms_step, avg = timed(K, prob, prec, args.steps, args.warmup)
     value = args.n / (ms_step * 1e-3) / 1e6
-    avg = {k: float(np.mean([p[k] for p in phases])) for k in phases[0]}
 
     # one step with the copy back, for the PCIe-inclusive figure and validation
     pot_leaf, tc = ev.evaluate(copy=True)
@@ -144,13 +180,18 @@ def main():
                 "algorithmic": f"{P2P_FLOPS_PER_PAIR:g} flop/pair x {wc['u_pairs']:.4g} U-list pairs per launch",
                 "kernel_ms": p2p_ms}
 
+    mixed = None
+    if not args.no_mixed_compare:
+        del ev
+        mixed = mixed_vs_fp64(K, max(5, args.steps // 2), args.warmup, args.nv)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
         cpu = cpu_baseline(args, threads)
 
     out = {
-        "metric": "Mpoints/s per FMM eval (4M uniform), FP64 GPU path",
+        "metric": "Mpoints/s per FMM eval (4M uniform), mixed vs FP64 GPU at matched sampled rel err",
         "value": value,
         "unit": "Mpoints/s",
         "n_gpus": 1,
@@ -160,11 +201,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded drand48 points and densities)",
         "config": {"workload": f"{args.dist} {args.n} points, separate targets, seed {args.seed}, "
-                               f"NP={args.np_}, {args.q} pts/box, FP64",
-                   "n_points": args.n, "np": args.np_, "pts_per_box": args.q, "precision": "fp64"},
+                               f"NP={args.np_}, {args.q} pts/box, {args.precision}",
+                   "n_points": args.n, "np": args.np_, "pts_per_box": args.q, "precision": args.precision},
         "phases_ms": {k: round(v, 4) for k, v in avg.items()},
         "mpoints_s_incl_d2h": args.n / ((ms_step + tc["copy"]) * 1e-3) / 1e6,
         "d2h_ms": tc["copy"],
@@ -172,6 +213,7 @@ def main():
         "setup_s": {"host_tree_lists": setup["tree_lists"], "generate": setup["generate"],
                     "operators": setup["operators"], "upload": t_create, "total_host": t_setup},
         "work": {k: wc[k] for k in ("u_pairs", "s2m_pairs", "l2t_pairs", "v_entries", "v_cmac", "gemm_flops")},
+        "mixed_vs_fp64": mixed,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

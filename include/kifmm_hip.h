@@ -26,13 +26,22 @@ typedef struct kifmm_gpu kifmm_gpu;
 
 enum {
   KIFMM_PREC_FP64 = 0, /* plain FP64 path */
-  KIFMM_PREC_MIXED = 1 /* NP <= 4: FP32 seed + FP64 Newton P2P, FP32 complex M2L */
+  KIFMM_PREC_MIXED = 1 /* NP <= 4: FP32 Gram-form distances on MFMA with FP64 recompute of
+                          close pairs, FP32 rsqrt, FP64 accumulation; FP32 S2M / L2T;
+                          FP64 pseudo-inverse applies and V-list */
+};
+
+enum {
+  KIFMM_GPU_GRAPH = 1 /* capture the whole evaluation in one HIP graph at the first full
+                         evaluate, replay it afterwards (phase times then report only
+                         total and copy) */
 };
 
 typedef struct kifmm_gpu_options {
   int device;    /* HIP device ordinal */
   int precision; /* KIFMM_PREC_* */
-  int reserved[6];
+  int flags;     /* KIFMM_GPU_* */
+  int reserved[5];
 } kifmm_gpu_options;
 
 /* Milliseconds, measured with HIP events on the handle's stream. */

@@ -21,6 +21,9 @@ extern "C" {
  * out[4*l + r] is accumulator element r of lane l. */
 int kifmm_test_mfma_layout(const double* a, const double* b, double* out);
 
+/* v_mfma_f32_16x16x4f32 on one wave, same convention (FP32). */
+int kifmm_test_mfma_layout_f32(const float* a, const float* b, float* out);
+
 /* C[:, j] = (acc ? C[:, j] : 0) + 2^sexp[j] * A * B[:, j] through the
  * evaluator's FP64 MFMA GEMM; A [M x K], B [K x ncols], C [M x ncols],
  * all column-major with leading dimensions M, K, M. */

@@ -198,6 +198,173 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
   }
 }
 
+/* ------------------------------------------------- U-list, mixed precision */
+
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+/* Mixed-precision U-list (NP <= 4).  Coordinates are centred on the
+ * target leaf and scaled by 1/h (h = its half-width, a power of two, so
+ * the scaling is exact); the distances of a 16 x 16 (target x source) tile
+ * come from one v_mfma_f32_16x16x4f32 on the Gram form
+ *     r^2 = |t|^2 + [-2tx, -2ty, -2tz, 1] . [sx, sy, sz, |s|^2]
+ * (A[i][k] in lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane
+ * j + 16(i/4), element i%4: measured by test_mfma_f32_operand_layout, C
+ * preloaded with |t|^2), then w * v_rsq_f32(r^2) is summed in FP32 over
+ * a source tile and folded into FP64 accumulators.  The FP32 Gram r^2 has
+ * an absolute error near 4 eps32 (r + 2 sqrt(3))^2, so pairs closer than
+ * MIX_TAU h (Gram r^2 < MIX_TAU^2) are recomputed in FP64 from the FP64
+ * centred coordinates and added to the FP64 sums (r = 0 exactly 0 for
+ * coincident points); every other pair's 1/r is within ~2e-4 relative, a
+ * contribution below 1e-6 of a target's U-list sum. */
+constexpr float MIX_TAU2 = 0.01f;
+constexpr double MIX_FAR = 1.0e6;
+
+template <bool MASK, int G>
+__device__ __forceinline__ void mixed_chunks(const float4* __restrict__ sB, const float* __restrict__ sW,
+                                             const double4* __restrict__ sS, const double4* const* sTg,
+                                             int nchunks, const float* a, const v4f* c, double (*acc)[4],
+                                             int lane) {
+  const int j = lane & 15, k4 = lane >> 4;
+  const float* sBf = (const float*)sB;
+  float part[G][4];
+#pragma unroll
+  for (int q = 0; q < G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) part[q][e] = 0.0f;
+#pragma unroll 2
+  for (int ch = 0; ch < nchunks; ch++) {
+    const int s = ch * 16 + j;
+    const float b = sBf[4 * s + k4];
+    const float w = sW[s];
+    v4f d[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) d[q] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b, c[q], 0, 0, 0);
+    bool close = false;
+#pragma unroll
+    for (int q = 0; q < G; q++)
+      close |= (d[q][0] < MIX_TAU2) | (d[q][1] < MIX_TAU2) | (d[q][2] < MIX_TAU2) | (d[q][3] < MIX_TAU2);
+    if (__any(close)) {
+      if (close) {
+        const double4 sp = sS[s];
+#pragma unroll
+        for (int q = 0; q < G; q++)
+#pragma unroll
+          for (int e = 0; e < 4; e++) {
+            if (d[q][e] < MIX_TAU2) {
+              const double4 tp = sTg[q][4 * k4 + e];
+              const double dx = tp.x - sp.x, dy = tp.y - sp.y, dz = tp.z - sp.z;
+              acc[q][e] = __builtin_fma(sp.w, inv_r<MASK>(__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx))),
+                                        acc[q][e]);
+              d[q][e] = __builtin_inff();
+            }
+          }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < G; q++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) part[q][e] = __builtin_fmaf(w, __builtin_amdgcn_rsqf(d[q][e]), part[q][e]);
+  }
+#pragma unroll
+  for (int q = 0; q < G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[q][e] += (double)part[q][e];
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                         const int* __restrict__ rng_ptr,
+                                                         const int2* __restrict__ rng,
+                                                         const double4* __restrict__ geo,
+                                                         const double4* __restrict__ src4,
+                                                         const double4* __restrict__ trg4,
+                                                         double* __restrict__ pot) {
+  __shared__ float4 sB[P2P_TILE];
+  __shared__ float sW[P2P_TILE];
+  __shared__ double4 sS[P2P_TILE];
+  __shared__ double4 sT[P2P_MAXT];
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const int4 it = items[item];
+  const double4 g = geo[it.w];
+  const double ih = 1.0 / g.w;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ngroups = (it.z + 15) >> 4;
+  /* targets in leaf units; slots past the count repeat the last target */
+  for (int k = tid; k < ngroups * 16; k += 256) {
+    const double4 q = trg4[it.y + min(k, it.z - 1)];
+    const double x = (q.x - g.x) * ih, y = (q.y - g.y) * ih, z = (q.z - g.z) * ih;
+    sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+  }
+  __syncthreads();
+  float a[2];
+  v4f c[2];
+  double acc[2][4];
+  const double4* tg[2];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int grp = min(wave + 4 * q, ngroups - 1);
+    tg[q] = sT + 16 * grp;
+    const double4 t = tg[q][lane & 15];
+    const int k4 = lane >> 4;
+    a[q] = (float)(k4 == 0 ? -2.0 * t.x : k4 == 1 ? -2.0 * t.y : k4 == 2 ? -2.0 * t.z : 1.0);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      c[q][e] = (float)tg[q][4 * k4 + e].w;
+      acc[q][e] = 0.0;
+    }
+  }
+  int ri = rng_ptr[it.x], ro = 0;
+  const int rlast = rng_ptr[it.x + 1];
+  while (ri < rlast) {
+    __syncthreads();
+    int fill = 0;
+    while (fill < P2P_TILE && ri < rlast) {
+      const int2 r = rng[ri];
+      const int take = min(r.y - ro, P2P_TILE - fill);
+      for (int k = tid; k < take; k += 256) {
+        const double4 q = src4[r.x + ro + k];
+        const double x = (q.x - g.x) * ih, y = (q.y - g.y) * ih, z = (q.z - g.z) * ih;
+        sS[fill + k] = make_double4(x, y, z, q.w);
+        sB[fill + k] = make_float4((float)x, (float)y, (float)z, (float)__builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+        sW[fill + k] = (float)q.w;
+      }
+      fill += take;
+      ro += take;
+      if (ro == r.y) {
+        ri++;
+        ro = 0;
+      }
+    }
+    const int ns = (fill + 15) & ~15;
+    for (int k = fill + tid; k < ns; k += 256) {
+      sS[k] = make_double4(MIX_FAR, MIX_FAR, MIX_FAR, 0.0);
+      sB[k] = make_float4((float)MIX_FAR, (float)MIX_FAR, (float)MIX_FAR, (float)(3.0 * MIX_FAR * MIX_FAR));
+      sW[k] = 0.0f;
+    }
+    __syncthreads();
+    if (wave + 4 < ngroups) mixed_chunks<MASK, 2>(sB, sW, sS, tg, ns >> 4, a, c, acc, lane);
+    else if (wave < ngroups) mixed_chunks<MASK, 1>(sB, sW, sS, tg, ns >> 4, a, c, acc, lane);
+  }
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int grp = wave + 4 * q;
+    if (grp < ngroups) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        double v = acc[q][e];
+        v += __shfl_xor(v, 8, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 1, 64);
+        const int t = 16 * grp + 4 * (lane >> 4) + e;
+        if ((lane & 15) == 0 && t < it.z) pot[it.y + t] = OOFP_R * ih * v;
+      }
+    }
+  }
+}
+
 /* ------------------------------------------------------------------ S2M */
 
 /* Lanes as targets, sources wave-uniform.  Every lane of the wave walks the
@@ -244,6 +411,72 @@ static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ ite
   const double tz = g.z + g.w * surf_uc[2 * ld_uc_surf + jj];
   const double acc = sum_uniform<false>(tx, ty, tz, src4 + it.y, it.z);
   if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * acc;
+}
+
+/* Mixed-precision S2M and L2T (NP <= 4): the same lane-per-target scheme
+ * in FP32, in units of the leaf's half-width h centred on the leaf (the
+ * check and equivalent surfaces are unit-box surfaces there, shared by all
+ * leaves), summed in FP32 per lane and scaled by 1/h into the FP64 check
+ * values or potentials.  Surface points and sources are at least 1.27 h
+ * apart, so FP32 differences carry ~1e-7 relative error; the pinv applies
+ * that follow stay FP64. */
+static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items,
+                                                          const double4* __restrict__ geo,
+                                                          const double* __restrict__ surf_uc, int n_uc,
+                                                          int ld_uc_surf, const double4* __restrict__ src4,
+                                                          double* __restrict__ chk, int ld_chk) {
+  const int4 it = items[blockIdx.x];
+  const int j = threadIdx.x, jj = min(j, n_uc - 1);
+  const double4 g = geo[it.x];
+  const double ih = 1.0 / g.w;
+  const float tx = (float)surf_uc[jj], ty = (float)surf_uc[ld_uc_surf + jj], tz = (float)surf_uc[2 * ld_uc_surf + jj];
+  const double4* __restrict__ src = src4 + it.y;
+  float acc0 = 0.0f, acc1 = 0.0f;
+  int k = 0;
+  for (; k + 2 <= it.z; k += 2) {
+    const double4 s0 = src[k], s1 = src[k + 1];
+    const float ax = tx - (float)((s0.x - g.x) * ih), ay = ty - (float)((s0.y - g.y) * ih),
+                az = tz - (float)((s0.z - g.z) * ih);
+    const float bx = tx - (float)((s1.x - g.x) * ih), by = ty - (float)((s1.y - g.y) * ih),
+                bz = tz - (float)((s1.z - g.z) * ih);
+    acc0 = __builtin_fmaf((float)s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    acc1 = __builtin_fmaf((float)s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+  }
+  if (k < it.z) {
+    const double4 s0 = src[k];
+    const float ax = tx - (float)((s0.x - g.x) * ih), ay = ty - (float)((s0.y - g.y) * ih),
+                az = tz - (float)((s0.z - g.z) * ih);
+    acc0 = __builtin_fmaf((float)s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+  }
+  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
+}
+
+static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items,
+                                                         const double4* __restrict__ geo,
+                                                         const double* __restrict__ surf_de, int ld_surf, int n_e,
+                                                         const double* __restrict__ dden, int ld_e,
+                                                         const double4* __restrict__ trg4, double* __restrict__ pot) {
+  const int4 it = items[blockIdx.x];
+  const int k = threadIdx.x;
+  const double4 q = trg4[it.y + min(k, it.z - 1)];
+  const double4 g = geo[it.x];
+  const double ih = 1.0 / g.w;
+  const float tx = (float)((q.x - g.x) * ih), ty = (float)((q.y - g.y) * ih), tz = (float)((q.z - g.z) * ih);
+  const double* __restrict__ den = dden + (size_t)it.x * ld_e;
+  float acc0 = 0.0f, acc1 = 0.0f;
+  int j = 0;
+  for (; j + 2 <= n_e; j += 2) {
+    const float ax = tx - (float)surf_de[j], ay = ty - (float)surf_de[ld_surf + j], az = tz - (float)surf_de[2 * ld_surf + j];
+    const float bx = tx - (float)surf_de[j + 1], by = ty - (float)surf_de[ld_surf + j + 1],
+                bz = tz - (float)surf_de[2 * ld_surf + j + 1];
+    acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    acc1 = __builtin_fmaf((float)den[j + 1], __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+  }
+  if (j < n_e) {
+    const float ax = tx - (float)surf_de[j], ay = ty - (float)surf_de[ld_surf + j], az = tz - (float)surf_de[2 * ld_surf + j];
+    acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+  }
+  if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((double)acc0 + (double)acc1);
 }
 
 /* ------------------------------------------------------------ GEMM (MFMA) */

@@ -89,6 +89,11 @@ struct kifmm_gpu {
   double* ws = nullptr;
   size_t ws_len = 0;
   kifmm_work_counts counts{};
+  /* HIP graph of the whole launch sequence (KIFMM_GPU_GRAPH) */
+  int use_graph = 0;
+  bool capturing = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
 };
 
 namespace {
@@ -252,7 +257,7 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
   };
 
   /* U-list: merged source ranges per target leaf; a leaf's targets split
-   * into the fewest items of <= P2P_MAXT, balanced, in multiples of 8 */
+   * into the fewest items of <= P2P_MAXT, balanced, in multiples of 16 */
   P.rng_ptr.assign(1, 0);
   for (int i = 0; i < nn; i++) {
     if (t->child[i] != -1 || t->trg_num[i] <= 0) continue;
@@ -268,7 +273,7 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     P.rng_ptr.push_back((int)P.rng.size());
     const int n = t->trg_num[i];
     const int nit = (n + P2P_MAXT - 1) / P2P_MAXT;
-    const int per = (((n + nit - 1) / nit) + 7) & ~7;
+    const int per = (((n + nit - 1) / nit) + 15) & ~15;
     for (int c = 0; c < n; c += per)
       P.p2p_items.push_back(make_int4(set, t->trg_beg[i] + c, std::min(per, n - c), i));
   }
@@ -608,6 +613,11 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     }                                                                            \
   } while (0)
 
+#define REC(e)                                                   \
+  do {                                                           \
+    if (!g->capturing) HIP_TRY(hipEventRecord((e), g->stream)); \
+  } while (0)
+
 int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda, const double* in, double* out,
          int accumulate) {
   if (L.n == 0) return KIFMM_OK;
@@ -638,15 +648,15 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
     hipLaunchKernelGGL(k_dft_fwd<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->depth,
                        g->uden, g->ld_e, g->tw, g->cube2surf, g->X);
   LAUNCH_CHECK();
-  HIP_TRY(hipEventRecord(e1, g->stream));
+  REC(e1);
   hipLaunchKernelGGL(k_trans<NP>, dim3(343), dim3(256), 0, g->stream, g->tw, g->T);
   LAUNCH_CHECK();
-  HIP_TRY(hipEventRecord(e2, g->stream));
+  REC(e2);
   if (g->n_hgroups > 0)
     hipLaunchKernelGGL(k_hadamard<NP>, dim3(g->n_hgroups, g->F / HAD_FB), dim3(128), 0, g->stream, g->hgroups, g->huslots,
                        g->hloc, g->hslot, g->T, g->X, g->Y);
   LAUNCH_CHECK();
-  HIP_TRY(hipEventRecord(e3, g->stream));
+  REC(e3);
   if (g->n_vblocks > 0)
     hipLaunchKernelGGL(k_dft_inv<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->Y,
                        g->tw, g->surf2cube, g->dchk, g->ld_e);
@@ -658,8 +668,12 @@ template <bool MASK>
 int point_phases_u(kifmm_gpu* g) {
   if (g->n_p2p > 0) {
     const int per_xcd = (g->n_p2p + 7) / 8;
-    hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items, g->n_p2p,
-                       per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
+    if (g->precision == KIFMM_PREC_MIXED)
+      hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
+                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
+    else
+      hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items, g->n_p2p,
+                         per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
   }
   LAUNCH_CHECK();
   return KIFMM_OK;
@@ -672,7 +686,7 @@ int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
                        g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_ue, g->ld_surf, g->n_e, g->uden,
                        g->ld_e, g->src4, g->trg4, g->pot);
   LAUNCH_CHECK();
-  HIP_TRY(hipEventRecord(ew, g->stream));
+  REC(ew);
   if ((mask & KIFMM_PHASE_X) && g->n_xnodes > 0)
     hipLaunchKernelGGL(k_xlist<MASK>, dim3(g->n_xnodes), dim3(256), 0, g->stream, g->xnodes, g->xptr, g->xidx,
                        g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_dc, g->ld_surf, g->n_e, g->dchk,
@@ -681,11 +695,11 @@ int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
   return KIFMM_OK;
 }
 
-int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* times) {
+/* The launch sequence of one evaluation on the handle's stream.  Phase
+ * events are recorded only outside graph capture. */
+int enqueue(kifmm_gpu* g, unsigned mask) {
   hipEvent_t* ev = g->ev;
   const int lde = g->ld_e, lduc = g->ld_uc;
-  HIP_TRY(hipSetDevice(g->dev));
-  HIP_TRY(hipEventRecord(ev[0], g->stream));
   /* zero the accumulators (a fill kernel, so the sequence stays capturable) */
   TRY(fill(g, g->pot, (size_t)g->n_trg));
   TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
@@ -694,8 +708,12 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
-    hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->geo,
-                       g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    if (g->precision == KIFMM_PREC_MIXED)
+      hipLaunchKernelGGL(k_s2m_mixed, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
+                         g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    else
+      hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
+                         g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
     LAUNCH_CHECK();
     TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
@@ -703,10 +721,10 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
     TRY(gemm(g, g->lv[d].m2m_b, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
-  HIP_TRY(hipEventRecord(ev[1], g->stream));
+  REC(ev[1]);
   /* U-list */
   if (mask & KIFMM_PHASE_U) TRY(g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g));
-  HIP_TRY(hipEventRecord(ev[2], g->stream));
+  REC(ev[2]);
   /* V-list */
   if (mask & KIFMM_PHASE_V) switch (g->np) {
     case 3: TRY(vlist_phase<3>(g, ev[3], ev[4], ev[5])); break;
@@ -715,17 +733,17 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     default: set_error("NP must be 3, 4 or 6 on the GPU"); return KIFMM_EUNSUP;
   }
   else {
-    HIP_TRY(hipEventRecord(ev[3], g->stream));
-    HIP_TRY(hipEventRecord(ev[4], g->stream));
-    HIP_TRY(hipEventRecord(ev[5], g->stream));
+    REC(ev[3]);
+    REC(ev[4]);
+    REC(ev[5]);
   }
-  HIP_TRY(hipEventRecord(ev[6], g->stream));
+  REC(ev[6]);
   /* W and X */
   if (mask & (KIFMM_PHASE_W | KIFMM_PHASE_X))
     TRY(g->coincident ? point_phases_wx<true>(g, mask, ev[7]) : point_phases_wx<false>(g, mask, ev[7]));
   else
-    HIP_TRY(hipEventRecord(ev[7], g->stream));
-  HIP_TRY(hipEventRecord(ev[8], g->stream));
+    REC(ev[7]);
+  REC(ev[8]);
   /* down: per level L2L (depth >= 3) then DC2DE (depth >= 2), then L2T */
   for (int d = 2; d <= g->max_depth && (mask & KIFMM_PHASE_DOWN); d++) {
     if (d >= 3)
@@ -733,10 +751,39 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
-    hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de, g->ld_surf,
-                       g->n_e, g->dden, lde, g->trg4, g->pot);
+    hipLaunchKernelGGL(g->precision == KIFMM_PREC_MIXED ? k_l2t_mixed : k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream,
+                       g->l2t_items, g->geo, g->surf_de, g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
   LAUNCH_CHECK();
-  HIP_TRY(hipEventRecord(ev[9], g->stream));
+  REC(ev[9]);
+  return KIFMM_OK;
+}
+
+int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* times) {
+  hipEvent_t* ev = g->ev;
+  HIP_TRY(hipSetDevice(g->dev));
+  const bool graph = g->use_graph && mask == KIFMM_PHASE_ALL;
+  if (graph && !g->gexec) {
+    g->capturing = true;
+    HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
+    const int r = enqueue(g, mask);
+    hipGraph_t gr = nullptr;
+    const hipError_t e = hipStreamEndCapture(g->stream, &gr);
+    g->capturing = false;
+    if (r != KIFMM_OK) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return r;
+    }
+    HIP_TRY(e);
+    g->graph = gr;
+    HIP_TRY(hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0));
+  }
+  HIP_TRY(hipEventRecord(ev[0], g->stream));
+  if (graph) {
+    HIP_TRY(hipGraphLaunch(g->gexec, g->stream));
+    for (int i = 1; i <= 9; i++) HIP_TRY(hipEventRecord(ev[i], g->stream));
+  } else {
+    TRY(enqueue(g, mask));
+  }
   if (pot_leaf)
     HIP_TRY(hipMemcpyAsync(pot_leaf, g->pot, sizeof(double) * g->n_trg, hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipEventRecord(ev[10], g->stream));
@@ -760,6 +807,11 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     times->down = ms[9];
     times->copy = ms[10];
     times->total = tot;
+    if (graph) { /* one graph launch: no per-phase events inside it */
+      std::memset(times, 0, sizeof *times);
+      times->copy = ms[10];
+      times->total = tot;
+    }
   }
   return KIFMM_OK;
 }
@@ -805,8 +857,9 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
   }
   g->dev = opts ? opts->device : 0;
   g->precision = opts ? opts->precision : KIFMM_PREC_FP64;
-  if (g->precision != KIFMM_PREC_FP64) {
-    set_error("kifmm_gpu_create: only the FP64 path is available in this build");
+  g->use_graph = opts ? (opts->flags & KIFMM_GPU_GRAPH) != 0 : 0;
+  if (g->precision != KIFMM_PREC_FP64 && !(g->precision == KIFMM_PREC_MIXED && tree->np <= 4)) {
+    set_error("kifmm_gpu_create: precision must be FP64, or MIXED with NP <= 4");
     delete g;
     return KIFMM_EUNSUP;
   }
@@ -867,6 +920,8 @@ int kifmm_gpu_destroy(kifmm_gpu* g) {
   if (!g) return KIFMM_OK;
   (void)hipSetDevice(g->dev);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
   for (void* p : g->allocs) (void)hipFree(p);
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);

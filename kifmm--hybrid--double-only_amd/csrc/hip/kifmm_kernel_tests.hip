@@ -191,6 +191,32 @@ int trans_np(double* out) {
 
 extern "C" {
 
+namespace {
+__global__ void k_mfma32_probe(const float* a, const float* b, float* out) {
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int l = threadIdx.x;
+  v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[l], b[l], c, 0, 0, 0);
+  for (int r = 0; r < 4; r++) out[4 * l + r] = c[r];
+}
+}  // namespace
+
+int kifmm_test_mfma_layout_f32(const float* a, const float* b, float* out) {
+  if (!a || !b || !out) return KIFMM_EINVAL;
+  DevBufs B;
+  float* da = B.up(a, 64);
+  float* db = B.up(b, 64);
+  float* dout = B.up<float>(nullptr, 256);
+  if (!da || !db || !dout) {
+    set_error("device allocation failed");
+    return KIFMM_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_mfma32_probe, dim3(1), dim3(64), 0, 0, da, db, dout);
+  int r = finish("k_mfma32_probe");
+  if (r != KIFMM_OK) return r;
+  return down(out, dout, 256);
+}
+
 int kifmm_test_mfma_layout(const double* a, const double* b, double* out) {
   if (!a || !b || !out) return KIFMM_EINVAL;
   DevBufs B;

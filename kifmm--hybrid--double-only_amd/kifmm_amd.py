@@ -23,6 +23,7 @@ KIFMM_SRCNODE = 1
 KIFMM_TRGNODE = 2
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EUNSUP"}
 PREC_FP64, PREC_MIXED = 0, 1
+GPU_GRAPH = 1
 PHASE_UP, PHASE_U, PHASE_V, PHASE_W, PHASE_X, PHASE_DOWN, PHASE_ALL = 1, 2, 4, 8, 16, 32, 63
 
 
@@ -64,7 +65,7 @@ class ProblemOpts(C.Structure):
 
 
 class GpuOptions(C.Structure):
-    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("reserved", C.c_int * 6)]
+    _fields_ = [("device", C.c_int), ("precision", C.c_int), ("flags", C.c_int), ("reserved", C.c_int * 5)]
 
 
 class PhaseTimes(C.Structure):
@@ -255,11 +256,14 @@ class Problem:
         return dict(zip(("generate", "tree_lists", "operators"), t.tolist()))
 
 
-def gpu_plan(problem: Problem, view: TreeView | None = None):
-    """Host-only derivation and bounds check of the device work lists."""
+def gpu_plan(problem: Problem, view: TreeView | None = None, operators: Operators | None = None):
+    """Host-only derivation and bounds check of the device work lists
+    (operators only fix NP and the surface sizes here: any set of the same
+    NP serves)."""
     v = view if view is not None else problem.view()
     w = WorkCounts()
-    _check(lib.kifmm_gpu_plan(C.byref(v), C.byref(problem.operators()), C.byref(w)))
+    ops = operators if operators is not None else problem.operators()
+    _check(lib.kifmm_gpu_plan(C.byref(v), C.byref(ops), C.byref(w)))
     return w.as_dict()
 
 
@@ -270,11 +274,12 @@ def gpu_available() -> bool:
 class GpuEvaluator:
     """Device copy of one problem (kifmm_gpu_create) and its evaluation."""
 
-    def __init__(self, problem: Problem, device=0, precision=PREC_FP64, operators: Operators | None = None):
+    def __init__(self, problem: Problem, device=0, precision=PREC_FP64, operators: Operators | None = None,
+                 graph=False):
         self.problem = problem
         self._view = problem.view()
         self._ops = operators if operators is not None else problem.operators()
-        opts = GpuOptions(device, precision)
+        opts = GpuOptions(device, precision, GPU_GRAPH if graph else 0)
         h = C.c_void_p()
         _check(lib.kifmm_gpu_create(C.byref(self._view), C.byref(self._ops), C.byref(opts), C.byref(h)))
         self._h = h

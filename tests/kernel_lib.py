@@ -10,6 +10,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 for name, args in (
     ("kifmm_test_mfma_layout", [_dp, _dp, _dp]),
+    ("kifmm_test_mfma_layout_f32", [C.POINTER(C.c_float)] * 3),
     ("kifmm_test_gemm", [C.c_int, C.c_int, _dp, C.c_int, _dp, _ip, _dp, C.c_int]),
     ("kifmm_test_rsqrt", [C.c_int, _dp, _dp, _dp]),
     ("kifmm_test_dft_fwd", [C.c_int, C.c_int, _dp, _ip, _dp]),
@@ -34,6 +35,15 @@ def mfma_layout(a, b):
     b = np.ascontiguousarray(b, dtype=np.float64)
     out = np.zeros(256)
     K._check(lib.kifmm_test_mfma_layout(dp(a), dp(b), dp(out)))
+    return out.reshape(64, 4)
+
+
+def mfma_layout_f32(a, b):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    out = np.zeros(256, dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    K._check(lib.kifmm_test_mfma_layout_f32(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp)))
     return out.reshape(64, 4)
 
 

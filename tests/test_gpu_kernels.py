@@ -27,6 +27,19 @@ def test_mfma_f64_operand_layout():
     assert np.array_equal(out, cols + 1.0), f"column map differs:\n{out}"
 
 
+def test_mfma_f32_operand_layout():
+    """v_mfma_f32_16x16x4f32: A[i][k] in lane i + 16k, B[k][j] in lane j + 16k,
+    D[i][j] in lane j + 16(i//4), element i%4 (the FP32 output layout differs
+    from the FP64 one)."""
+    lanes = np.arange(64)
+    rows = 4 * (lanes // 16)[:, None] + np.arange(4)[None, :]
+    cols = np.repeat((lanes % 16)[:, None], 4, axis=1)
+    out = KL.mfma_layout_f32(lanes + 1.0, (lanes < 16).astype(float))   # D[i][j] = A[i][0] = i + 1
+    assert np.array_equal(out, rows + 1.0), f"row map differs:\n{out}"
+    out = KL.mfma_layout_f32((lanes < 16).astype(float), lanes + 1.0)   # D[i][j] = B[0][j] = j + 1
+    assert np.array_equal(out, cols + 1.0), f"column map differs:\n{out}"
+
+
 @pytest.mark.parametrize("M,Kd,n", [(56, 152, 37), (152, 448, 20), (448, 56, 33), (26, 104, 5), (296, 1216, 17),
                                     (152, 160, 64), (152, 448, 9000), (56, 1216, 8200), (98, 56, 300),
                                     (1216, 152, 700), (208, 26, 40)])
