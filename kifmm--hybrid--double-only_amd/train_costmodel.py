@@ -1,0 +1,128 @@
+#!/usr/bin/env python3
+"""Train the cost model on timings the FMM itself produces, then test it.
+
+Training set: 64 seeded inputs, N log-uniform in [2^18, 2^24] (NP = 6 capped
+at 2^22.5), distributions uniform cube / pole-clustered ellipsoid / mixture
+of 8 Gaussian clusters, NP in {3, 4, 6}.  Every input is built at every q in
+{32, ..., 1024} and timed on the GPU on every precision path its NP allows
+(FP64; mixed when NP <= 4); each row is (16 octree statistics, measured up /
+U / V / down ms).  The network is trained on the GPU by the HIP kernels
+(kifmm_mlp_train).  Held-out set: 16 further inputs, each swept exhaustively;
+the report gives, per input, the model's choice against the best of the
+sweep and against a fixed q = 64 on the FP64 path.
+
+    python train_costmodel.py --out DIR [--train 64] [--held 16]
+
+writes DIR/costmodel_weights.json, DIR/costmodel_data.json and
+DIR/costmodel_report.json and prints one line per input.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+sys.path.insert(0, str(Path(__file__).resolve().parent))
+import costmodel as CM  # noqa: E402
+import kifmm_amd as K  # noqa: E402
+
+DISTS = ("uniform", "ellipsoid", "gaussmix")
+
+
+def draw_inputs(count: int, seed0: int, rng_seed: int):
+    rng = np.random.default_rng(rng_seed)
+    out = []
+    for i in range(count):
+        dist = DISTS[i % 3]
+        np_ = (3, 4, 6)[(i // 3) % 3]
+        n = int(2.0 ** rng.uniform(18.0, 24.0 if np_ != 6 else 22.5))
+        out.append((n, dist, np_, seed0 + i))
+    return out
+
+
+def sweep(inp, qs, steps):
+    """Every (q, path) of one input: features and measured phase times."""
+    n, dist, np_, seed = inp
+    base = K.Problem.create(n, dist, 64, np_=np_, seed=seed)
+    pts = base.points()
+    pts = (pts["sx"], pts["sy"], pts["sz"], pts["den"], pts["tx"], pts["ty"], pts["tz"])
+    ops = base.operators()
+    rows = []
+    for q in qs:
+        prob = K.Problem.from_points(*pts, pts_max=q, np_=np_)
+        for prec in CM.paths_for(np_):
+            f = CM.features(prob, q, prec, ops)
+            ph, wall = CM.phase_times(prob, prec, steps=steps)
+            rows.append({"n": n, "dist": dist, "np": np_, "seed": seed, "q": q, "prec": prec,
+                         "features": [float(x) for x in f], "phases_ms": [float(x) for x in ph], "wall_ms": wall})
+        del prob
+    return rows, pts
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--out", required=True)
+    ap.add_argument("--train", type=int, default=64)
+    ap.add_argument("--held", type=int, default=16)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--epochs", type=int, default=4000)
+    ap.add_argument("--lr", type=float, default=3e-3)
+    args = ap.parse_args()
+    out = Path(args.out)
+    out.mkdir(parents=True, exist_ok=True)
+    qs = CM.Q_CHOICES
+    t0 = time.time()
+    data = []
+    for k, inp in enumerate(draw_inputs(args.train, 1000, 7)):
+        rows, _ = sweep(inp, qs, args.steps)
+        data += rows
+        best = min(rows, key=lambda r: r["wall_ms"])
+        print(f"train {k + 1}/{args.train} n={inp[0]} {inp[1]} NP={inp[2]}: best q={best['q']} "
+              f"prec={best['prec']} {best['wall_ms']:.2f} ms  [{time.time() - t0:.0f} s]", flush=True)
+    (out / "costmodel_data.json").write_text(json.dumps(data))
+    X = np.array([r["features"] for r in data])
+    Y = np.array([r["phases_ms"] for r in data])
+    model = CM.CostModel.fit(X, Y, epochs=args.epochs, lr=args.lr)
+    model.meta.update({"train_inputs": args.train, "q_choices": list(qs)})
+    model.save(out / "costmodel_weights.json")
+    print(f"trained on {len(data)} rows: loss {model.meta['first_loss']:.4f} -> {model.meta['final_loss']:.5f}",
+          flush=True)
+    tuner = CM.Tuner(model, qs)
+    report = []
+    for k, inp in enumerate(draw_inputs(args.held, 5000, 11)):
+        rows, pts = sweep(inp, qs, args.steps)
+        by = {(r["q"], r["prec"]): r for r in rows}
+        best = min(rows, key=lambda r: r["wall_ms"])
+        q, prec, pred, _ = tuner.choose(pts, inp[2])
+        chosen = by[(q, prec)]
+        default = by[(64, K.PREC_FP64)]
+        Xh = np.array([r["features"] for r in rows])
+        pred_all = model.predict_ms(Xh).sum(1)
+        meas_all = np.array([r["wall_ms"] for r in rows])
+        rec = {"n": inp[0], "dist": inp[1], "np": inp[2], "seed": inp[3],
+               "chosen": {"q": q, "prec": prec, "ms": chosen["wall_ms"], "predicted_ms": pred},
+               "best": {"q": best["q"], "prec": best["prec"], "ms": best["wall_ms"]},
+               "default_q64_fp64_ms": default["wall_ms"],
+               "chosen_over_best": chosen["wall_ms"] / best["wall_ms"],
+               "gain_over_default": default["wall_ms"] / chosen["wall_ms"],
+               "median_abs_log_error": float(np.median(np.abs(np.log(pred_all / meas_all))))}
+        report.append(rec)
+        print(f"held {k + 1}/{args.held} n={inp[0]} {inp[1]} NP={inp[2]}: model q={q} prec={prec} "
+              f"{chosen['wall_ms']:.2f} ms, best q={best['q']} prec={best['prec']} {best['wall_ms']:.2f} ms, "
+              f"q=64 FP64 {default['wall_ms']:.2f} ms", flush=True)
+    summ = {"held_out": report,
+            "geomean_chosen_over_best": float(np.exp(np.mean(np.log([r["chosen_over_best"] for r in report])))),
+            "geomean_gain_over_default": float(np.exp(np.mean(np.log([r["gain_over_default"] for r in report])))),
+            "exact_pick_fraction": float(np.mean([r["chosen_over_best"] <= 1.0 + 1e-9 for r in report])),
+            "model": model.meta}
+    (out / "costmodel_report.json").write_text(json.dumps(summ, indent=1))
+    print(json.dumps({k: v for k, v in summ.items() if k != "held_out"}), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())
