@@ -44,7 +44,7 @@ def parse():
     ap.add_argument("--q", type=int, default=512)
     ap.add_argument("--np", dest="np_", type=int, default=6)
     ap.add_argument("--seed", type=int, default=1)
-    ap.add_argument("--nv", type=int, default=200, help="validation samples (0 = skip)")
+    ap.add_argument("--nv", type=int, default=1000, help="validation samples, direct sums on the GPU (0 = skip)")
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64")
     ap.add_argument("--no-mixed-compare", action="store_true",
                     help="skip the mixed vs FP64 comparison on 4M uniform points at NP = 4")
@@ -124,10 +124,19 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     prob = K.Problem.create(n, "uniform", 256, np_=4, seed=1, coincide=0)
     prob.operators()
     out = {"workload": f"uniform {n} points, separate targets, seed 1, NP=4, 256 pts/box"}
+    sample = None  # one NV-point sample for every path (matched error)
     for name, prec, graph in (("fp64", K.PREC_FP64, False), ("fp64_graph", K.PREC_FP64, True),
                               ("mixed", K.PREC_MIXED, False), ("mixed_graph", K.PREC_MIXED, True)):
         ev, ms, avg = timed(K, prob, prec, steps, warmup, graph)
-        err = prob.validate(prob.scatter(ev.evaluate(copy=True)[0]), nv)[0] if nv > 0 else None
+        err = None
+        if nv > 0:
+            pot = prob.scatter(ev.evaluate(copy=True)[0])
+            if sample is None:
+                err, ids, direct = prob.validate_gpu(ev, pot, nv)
+                sample = (ids, direct)
+            else:
+                f = pot[sample[0]]
+                err = float(np.linalg.norm(sample[1] - f) / np.linalg.norm(f))
         out[name] = {"mpoints_s": n / (ms * 1e-3) / 1e6, "ms_per_step": ms, "sampled_rel_err": err}
         if not graph:
             out[name]["phases_ms"] = {k: round(v, 4) for k, v in avg.items()}
@@ -170,7 +179,7 @@ def main():
     pot = prob.scatter(pot_leaf)
     err = None
     if args.nv > 0:
-        err, _, _ = prob.validate(pot, args.nv)
+        err, _, _ = prob.validate_gpu(ev, pot, args.nv)
 
     p2p_ms = avg["p2p_kernel"]
     achieved = P2P_FLOPS_PER_PAIR * wc["u_pairs"] / (p2p_ms * 1e-3) / 1e12

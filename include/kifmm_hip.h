@@ -115,6 +115,18 @@ int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long
 /* 1 if a HIP device is usable from this process. */
 int kifmm_gpu_available(void);
 
+/* Direct FP64 sum phi(t) = 1/(4 pi) sum_s w_s / |t - s| (r = 0 excluded) at
+ * nt host-given points over all the handle's sources, on its device, with
+ * compensated summation.  Replaces the O(NV N) host loop of validate
+ * (fmm/driver.cc:141-206). */
+int kifmm_gpu_direct(kifmm_gpu* g, int nt, const double* tx, const double* ty, const double* tz, double* out);
+
+/* kifmm_problem_validate with the direct sums on the GPU: the same sample
+ * (the problem's generator stream continued) and the same relative L2
+ * error; g must hold p's sources.  Returns the error, or -1 on failure. */
+double kifmm_problem_validate_gpu(kifmm_problem* p, kifmm_gpu* g, const double* pot_orig, int nv, int* chk_ids,
+                                  double* direct_out);
+
 #ifdef __cplusplus
 }
 #endif

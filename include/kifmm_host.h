@@ -100,6 +100,15 @@ kifmm_problem* kifmm_problem_from_points(int n, const double* sx, const double* 
                                          const double* tz, int pts_max, int np);
 void kifmm_problem_destroy(kifmm_problem* p);
 
+/* Point files (no counterpart in the reference, which only generates its
+ * inputs, fmm/driver.cc:264-267): 8-byte magic "KIFMMPT1", int64 n, int32
+ * has_targets, int32 0, then n little-endian doubles of sx, sy, sz, den and,
+ * with targets, tx, ty, tz; coordinates in [-1,1]^3.  tx = ty = tz = NULL
+ * writes a sources-are-targets file. */
+int kifmm_points_write(const char* path, int n, const double* sx, const double* sy, const double* sz,
+                       const double* den, const double* tx, const double* ty, const double* tz);
+kifmm_problem* kifmm_problem_from_file(const char* path, int pts_max, int np);
+
 int kifmm_problem_view(const kifmm_problem* p, kifmm_tree_view* out);
 /* Builds (once) and returns the operators; the arrays live in the problem. */
 int kifmm_problem_operators(kifmm_problem* p, kifmm_operators* out);

@@ -5,6 +5,9 @@
  *   fmmd--hip <#pts> {uniform,ellipseUniformAngles,ellipsoid,gaussmix} <pts-per-box>
  * environment NP (3, 4, 6), NV (validation samples), C (sources =
  * targets), SEED (default 26, the reference's srand48(26)).
+ * Additions: distribution "file:<path>" reads a KIFMMPT1 point file (the
+ * count argument is then ignored); PREC=mixed selects the mixed-precision
+ * path (NP <= 4); GRAPH=1 replays the evaluation as one HIP graph.
  * Steps: generate + trees + lists + gather, operators, kifmm_gpu_create
  * (preproc), kifmm_gpu_evaluate (run, with the copy back timed),
  * scatter to input order, validate, print the reference's estimates.
@@ -17,6 +20,11 @@
 
 #include "kifmm_hip.h"
 
+static int tv_count(kifmm_problem* p) {
+  kifmm_tree_view v;
+  return kifmm_problem_view(p, &v) == KIFMM_OK ? v.n_src : 0;
+}
+
 static void usage(const char* prog) { fprintf(stderr, "usage: %s <N> <distribution> <pts/box> \n", prog); }
 
 int main(int argc, char** argv) {
@@ -24,18 +32,20 @@ int main(int argc, char** argv) {
     usage(argv[0]);
     return -1;
   }
-  const int n = atoi(argv[1]);
+  int n = atoi(argv[1]);
   const char* dist = argv[2];
   const int q = atoi(argv[3]);
   kifmm_problem_opts po = {-1, -1, -1, 0};
   fprintf(stderr, "Building trees and interaction lists...\n");
-  kifmm_problem* p = kifmm_problem_create(n, dist, q, &po);
+  kifmm_problem* p = strncmp(dist, "file:", 5) == 0 ? kifmm_problem_from_file(dist + 5, q, kifmm_getenv_accuracy())
+                                                    : kifmm_problem_create(n, dist, q, &po);
   if (!p) {
     fprintf(stderr, "error: %s\n", kifmm_last_error());
     return 1;
   }
   kifmm_tree_view tv;
   kifmm_operators ops;
+  n = tv_count(p);
   fprintf(stderr, "Setup for translations....\n");
   if (kifmm_problem_view(p, &tv) || kifmm_problem_operators(p, &ops)) {
     fprintf(stderr, "error: %s\n", kifmm_last_error());
@@ -44,6 +54,10 @@ int main(int argc, char** argv) {
   fprintf(stderr, "Preprocessing...\n");
   kifmm_gpu_options go;
   memset(&go, 0, sizeof go);
+  const char* prec = getenv("PREC");
+  if (prec && strcmp(prec, "mixed") == 0) go.precision = KIFMM_PREC_MIXED;
+  const char* graph = getenv("GRAPH");
+  if (graph && atoi(graph) != 0) go.flags |= KIFMM_GPU_GRAPH;
   kifmm_gpu* g = nullptr;
   if (kifmm_gpu_create(&tv, &ops, &go, &g) != KIFMM_OK) {
     fprintf(stderr, "error: %s\n", kifmm_last_error());

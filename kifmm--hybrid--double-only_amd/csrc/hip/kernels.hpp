@@ -231,7 +231,6 @@ __device__ __forceinline__ void mixed_chunks(const float4* __restrict__ sB, cons
   for (int q = 0; q < G; q++)
 #pragma unroll
     for (int e = 0; e < 4; e++) part[q][e] = 0.0f;
-#pragma unroll 2
   for (int ch = 0; ch < nchunks; ch++) {
     const int s = ch * 16 + j;
     const float b = sBf[4 * s + k4];
@@ -1116,6 +1115,48 @@ static __global__ __launch_bounds__(64) void k_l2t(const int4* __restrict__ item
     acc0 = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
   }
   if (k < it.z) pot[it.y + k] += OOFP_R * (acc0 + acc1);
+}
+
+/* Sampled direct sum for validation (validate, fmm/driver.cc:141-206, on
+ * the GPU): one block per target point, 256 lanes stride over all sources,
+ * FP64 w / sqrt(r^2) (IEEE division and square root, r = 0 excluded) with
+ * Neumaier-compensated lane sums, combined by lane 0 in lane order (also
+ * compensated): the result is independent of the source order to about
+ * one rounding. */
+__device__ __forceinline__ void neumaier(double& s, double& c, double v) {
+  const double t = s + v;
+  c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+  s = t;
+}
+
+static __global__ __launch_bounds__(256) void k_direct(int nt, const double* __restrict__ tx,
+                                                       const double* __restrict__ ty, const double* __restrict__ tz,
+                                                       const double4* __restrict__ src4, int n,
+                                                       double* __restrict__ out) {
+  __shared__ double ss[256], sc[256];
+  const int tid = threadIdx.x;
+  for (int t = blockIdx.x; t < nt; t += gridDim.x) {
+    const double x = tx[t], y = ty[t], z = tz[t];
+    double s = 0.0, c = 0.0;
+    for (int j = tid; j < n; j += 256) {
+      const double4 p = src4[j];
+      const double dx = x - p.x, dy = y - p.y, dz = z - p.z;
+      const double r2 = dx * dx + dy * dy + dz * dz;
+      if (r2 != 0.0) neumaier(s, c, p.w / sqrt(r2));
+    }
+    ss[tid] = s;
+    sc[tid] = c;
+    __syncthreads();
+    if (tid == 0) {
+      double S = 0.0, Cc = 0.0;
+      for (int k = 0; k < 256; k++) {
+        neumaier(S, Cc, ss[k]);
+        Cc += sc[k];
+      }
+      out[t] = OOFP_R * (S + Cc);
+    }
+    __syncthreads();
+  }
 }
 
 static __global__ void k_fill(double* __restrict__ p, size_t n, double v) {

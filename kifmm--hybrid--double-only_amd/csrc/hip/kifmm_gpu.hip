@@ -930,6 +930,58 @@ int kifmm_gpu_destroy(kifmm_gpu* g) {
   return KIFMM_OK;
 }
 
+int kifmm_gpu_direct(kifmm_gpu* g, int nt, const double* tx, const double* ty, const double* tz, double* out) {
+  if (!g || nt <= 0 || !tx || !ty || !tz || !out) {
+    set_error("kifmm_gpu_direct: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  HIP_TRY(hipSetDevice(g->dev));
+  double* d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(double) * 4 * (size_t)nt));
+  auto done = [&](int r) {
+    (void)hipFree(d);
+    return r;
+  };
+  const size_t b = sizeof(double) * nt;
+  if (hipMemcpyAsync(d, tx, b, hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      hipMemcpyAsync(d + nt, ty, b, hipMemcpyHostToDevice, g->stream) != hipSuccess ||
+      hipMemcpyAsync(d + 2 * (size_t)nt, tz, b, hipMemcpyHostToDevice, g->stream) != hipSuccess) {
+    set_error("kifmm_gpu_direct: upload failed");
+    return done(KIFMM_EDEVICE);
+  }
+  hipLaunchKernelGGL(k_direct, dim3(std::min(nt, 65535)), dim3(256), 0, g->stream, nt, d, d + nt, d + 2 * (size_t)nt,
+                     g->src4, g->n_src, d + 3 * (size_t)nt);
+  if (hipGetLastError() != hipSuccess ||
+      hipMemcpyAsync(out, d + 3 * (size_t)nt, b, hipMemcpyDeviceToHost, g->stream) != hipSuccess ||
+      hipStreamSynchronize(g->stream) != hipSuccess) {
+    set_error("kifmm_gpu_direct: kernel or copy failed");
+    return done(KIFMM_EDEVICE);
+  }
+  return done(KIFMM_OK);
+}
+
+double kifmm_problem_validate_gpu(kifmm_problem* p, kifmm_gpu* g, const double* pot_orig, int nv, int* chk_ids,
+                                  double* direct_out) {
+  if (!p || !g || !pot_orig || nv <= 0) {
+    set_error("kifmm_problem_validate_gpu: invalid argument");
+    return -1.0;
+  }
+  std::vector<int> ids(nv);
+  std::vector<double> xyz(3 * (size_t)nv), direct(nv);
+  if (kifmm::sample_targets(p, nv, ids.data(), xyz.data()) != KIFMM_OK) return -1.0;
+  if (kifmm_gpu_direct(g, nv, xyz.data(), xyz.data() + nv, xyz.data() + 2 * (size_t)nv, direct.data()) != KIFMM_OK)
+    return -1.0;
+  double vn = 0.0, en = 0.0;
+  for (int k = 0; k < nv; k++) {
+    const double f = pot_orig[ids[k]];
+    vn += f * f;
+    en += (direct[k] - f) * (direct[k] - f);
+  }
+  if (chk_ids) std::memcpy(chk_ids, ids.data(), sizeof(int) * nv);
+  if (direct_out) std::memcpy(direct_out, direct.data(), sizeof(double) * nv);
+  return std::sqrt(en) / std::sqrt(vn);
+}
+
 int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifmm_work_counts* out) {
   if (!tree || !ops) {
     set_error("kifmm_gpu_plan: null argument");

@@ -655,6 +655,55 @@ kifmm_problem* kifmm_problem_from_points(int n, const double* sx, const double* 
   return checked_create(p);
 }
 
+/* Point files: 8-byte magic "KIFMMPT1", int64 n, int32 has_targets,
+ * int32 reserved (0), then n little-endian doubles each of sx, sy, sz, den
+ * and, when has_targets, tx, ty, tz.  Coordinates in the root box
+ * [-1,1]^3; without targets the sources are the targets (C = 1). */
+static const char kPtsMagic[8] = {'K', 'I', 'F', 'M', 'M', 'P', 'T', '1'};
+
+int kifmm_points_write(const char* path, int n, const double* sx, const double* sy, const double* sz,
+                       const double* den, const double* tx, const double* ty, const double* tz) {
+  if (!path || n <= 0 || !sx || !sy || !sz || !den || ((tx || ty || tz) && !(tx && ty && tz))) {
+    set_error("kifmm_points_write: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  FILE* f = std::fopen(path, "wb");
+  if (!f) { set_error(std::string("kifmm_points_write: cannot open ") + path); return KIFMM_EINVAL; }
+  const long long nn = n;
+  const int hdr[2] = {tx ? 1 : 0, 0};
+  bool ok = std::fwrite(kPtsMagic, 1, 8, f) == 8 && std::fwrite(&nn, sizeof nn, 1, f) == 1 &&
+            std::fwrite(hdr, sizeof(int), 2, f) == 2;
+  const double* arrs[7] = {sx, sy, sz, den, tx, ty, tz};
+  for (int k = 0; k < (tx ? 7 : 4) && ok; k++) ok = std::fwrite(arrs[k], sizeof(double), n, f) == (size_t)n;
+  ok = (std::fclose(f) == 0) && ok;
+  if (!ok) { set_error("kifmm_points_write: write failed"); return KIFMM_EINVAL; }
+  return KIFMM_OK;
+}
+
+kifmm_problem* kifmm_problem_from_file(const char* path, int pts_max, int np) {
+  if (!path) { set_error("kifmm_problem_from_file: null path"); return nullptr; }
+  FILE* f = std::fopen(path, "rb");
+  if (!f) { set_error(std::string("kifmm_problem_from_file: cannot open ") + path); return nullptr; }
+  char magic[8];
+  long long nn = 0;
+  int hdr[2] = {0, 0};
+  bool ok = std::fread(magic, 1, 8, f) == 8 && std::memcmp(magic, kPtsMagic, 8) == 0 &&
+            std::fread(&nn, sizeof nn, 1, f) == 1 && std::fread(hdr, sizeof(int), 2, f) == 2 && nn > 0 &&
+            nn <= (1LL << 31) - 1 && (hdr[0] == 0 || hdr[0] == 1);
+  const int n = ok ? (int)nn : 0;
+  std::vector<double> a[7];
+  for (int k = 0; k < (hdr[0] ? 7 : 4) && ok; k++) {
+    a[k].resize(n);
+    ok = std::fread(a[k].data(), sizeof(double), n, f) == (size_t)n;
+  }
+  std::fclose(f);
+  if (!ok) { set_error(std::string("kifmm_problem_from_file: malformed or truncated file ") + path); return nullptr; }
+  if (!hdr[0]) return kifmm_problem_from_points(n, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[0].data(),
+                                                a[1].data(), a[2].data(), pts_max, np);
+  return kifmm_problem_from_points(n, a[0].data(), a[1].data(), a[2].data(), a[3].data(), a[4].data(), a[5].data(),
+                                   a[6].data(), pts_max, np);
+}
+
 void kifmm_problem_destroy(kifmm_problem* p) { delete p; }
 
 int kifmm_problem_view(const kifmm_problem* p, kifmm_tree_view* v) {
@@ -713,6 +762,21 @@ int kifmm_problem_points(const kifmm_problem* p, double* sx, double* sy, double*
   if (tz) std::memcpy(tz, p->tz.data(), b);
   return KIFMM_OK;
 }
+
+}  // extern "C"
+
+int kifmm::sample_targets(kifmm_problem* p, int nv, int* ids, double* xyz) {
+  if (!p || nv <= 0 || !ids || !xyz) return KIFMM_EINVAL;
+  for (int k = 0; k < nv; k++) ids[k] = (int)std::floor(p->rng.next() * p->n);
+  for (int k = 0; k < nv; k++) {
+    xyz[k] = p->tx[ids[k]];
+    xyz[nv + k] = p->ty[ids[k]];
+    xyz[2 * nv + k] = p->tz[ids[k]];
+  }
+  return KIFMM_OK;
+}
+
+extern "C" {
 
 double kifmm_problem_validate(kifmm_problem* p, const double* pot_orig, int nv, int* chk_ids,
                               double* direct_out) {

@@ -102,6 +102,10 @@ _sig("kifmm_problem_operators", C.c_int, [C.c_void_p, C.POINTER(Operators)])
 _sig("kifmm_problem_scatter", C.c_int, [C.c_void_p, _dp, _dp])
 _sig("kifmm_problem_points", C.c_int, [C.c_void_p] + [_dp] * 7)
 _sig("kifmm_problem_validate", C.c_double, [C.c_void_p, _dp, C.c_int, _ip, _dp])
+_sig("kifmm_points_write", C.c_int, [C.c_char_p, C.c_int, _dp, _dp, _dp, _dp, _dp, _dp, _dp])
+_sig("kifmm_problem_from_file", C.c_void_p, [C.c_char_p, C.c_int, C.c_int])
+_sig("kifmm_problem_validate_gpu", C.c_double, [C.c_void_p, C.c_void_p, _dp, C.c_int, _ip, _dp])
+_sig("kifmm_gpu_direct", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp])
 _sig("kifmm_problem_setup_times", C.c_int, [C.c_void_p, _dp])
 for _n in ("accuracy", "validate", "coincide", "block_size", "upbs"):
     _sig(f"kifmm_getenv_{_n}", C.c_int, [])
@@ -250,6 +254,26 @@ class Problem:
             raise KifmmError(-1, last_error())
         return r, ids, direct
 
+    @classmethod
+    def from_file(cls, path, pts_max=64, np_=6):
+        """Points from a KIFMMPT1 file (kifmm_problem_from_file)."""
+        h = lib.kifmm_problem_from_file(str(path).encode(), pts_max, np_)
+        if not h:
+            raise KifmmError(-1, last_error())
+        return cls(h)
+
+    def validate_gpu(self, evaluator, pot_orig, nv=1000):
+        """validate() with the direct sums on the GPU (kifmm_problem_validate_gpu):
+        same sample, same error definition; `evaluator` holds this problem."""
+        pot_orig = np.ascontiguousarray(pot_orig, dtype=np.float64)
+        ids = np.zeros(nv, dtype=np.int32)
+        direct = np.zeros(nv)
+        r = lib.kifmm_problem_validate_gpu(self._h, evaluator._h, _dptr(pot_orig), nv, ids.ctypes.data_as(_ip),
+                                           _dptr(direct))
+        if r < 0:
+            raise KifmmError(-1, last_error())
+        return r, ids, direct
+
     def setup_times(self):
         t = np.zeros(3)
         _check(lib.kifmm_problem_setup_times(self._h, _dptr(t)))
@@ -265,6 +289,15 @@ def gpu_plan(problem: Problem, view: TreeView | None = None, operators: Operator
     ops = operators if operators is not None else problem.operators()
     _check(lib.kifmm_gpu_plan(C.byref(v), C.byref(ops), C.byref(w)))
     return w.as_dict()
+
+
+def write_points(path, sx, sy, sz, den, tx=None, ty=None, tz=None):
+    """Write a KIFMMPT1 point file (kifmm_points_write); no targets = the
+    sources are the targets."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (sx, sy, sz, den)]
+    t = [None] * 3 if tx is None else [np.ascontiguousarray(v, dtype=np.float64) for v in (tx, ty, tz)]
+    _check(lib.kifmm_points_write(str(path).encode(), len(a[0]), *[_dptr(v) for v in a],
+                                  *[(_dptr(v) if v is not None else None) for v in t]))
 
 
 def gpu_available() -> bool:

@@ -178,3 +178,25 @@ def test_from_points_matches_generated():
         assert np.array_equal(ta[k], tb[k]), k
     with pytest.raises(K.KifmmError):
         K.Problem.from_points(pts["sx"] * 3, pts["sy"], pts["sz"], pts["den"], pts_max=16, np_=4)
+
+
+def test_point_file_roundtrip(tmp_path):
+    """KIFMMPT1 files: separate targets and sources-are-targets; the tree
+    from the file equals the tree from the same points in memory; a
+    truncated file is an error, not a crash."""
+    a = K.Problem.create(5000, "gaussmix", 16, np_=4, seed=12)
+    pts = a.points()
+    f = tmp_path / "p.bin"
+    K.write_points(f, pts["sx"], pts["sy"], pts["sz"], pts["den"], pts["tx"], pts["ty"], pts["tz"])
+    b = K.Problem.from_file(f, pts_max=16, np_=4)
+    ta, tb = a.tree(), b.tree()
+    for k in KEYS:
+        assert np.array_equal(ta[k], tb[k]), k
+    g = tmp_path / "c.bin"
+    K.write_points(g, pts["sx"], pts["sy"], pts["sz"], pts["den"])
+    c = K.Problem.from_file(g, pts_max=16, np_=4)
+    assert c.view().coincident == 1 and np.array_equal(c.points()["tx"], pts["sx"])
+    h = tmp_path / "t.bin"
+    h.write_bytes(f.read_bytes()[:1000])
+    with pytest.raises(K.KifmmError):
+        K.Problem.from_file(h, pts_max=16, np_=4)
