@@ -7,6 +7,7 @@ import numpy as np
 import pytest
 
 import kifmm_amd as K
+import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
@@ -36,10 +37,12 @@ def test_uniform_4m_np6_linearity_and_error():
         e = K.GpuEvaluator(p)
         pots.append(p.scatter(e.evaluate()[0]))
         del e
-    # phi(w1) = phi(w1 - 2 w2) + 2 phi(w2): every phase is linear in w
+    # phi(w1) = phi(w1 - 2 w2) + 2 phi(w2): every phase is linear in w.  Each
+    # NP = 6 evaluation carries the pinv conditioning floor (oracle_lib.PARITY_RTOL,
+    # ~2e-11 measured on the oracle itself); the combination adds three of them.
     r = rel(pots[1] + 2.0 * pots[0], pot1)
     print(f"  linearity {r:.3e}", flush=True)
-    assert r < 1e-12
+    assert r < 3 * O.PARITY_RTOL[6]
 
 
 def test_ellipsoid_1m_np4_graph():
