@@ -419,40 +419,52 @@ static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ ite
  * values or potentials.  Surface points and sources are at least 1.27 h
  * apart, so FP32 differences carry ~1e-7 relative error; the pinv applies
  * that follow stay FP64. */
+/* FP32 leaf-local source records for the mixed S2M: (s - c) / h and w for
+ * every source of every source leaf (one block per leaf). */
+static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restrict__ items,
+                                                            const double4* __restrict__ geo,
+                                                            const double4* __restrict__ src4,
+                                                            float4* __restrict__ src32) {
+  const int4 it = items[blockIdx.x];
+  const double4 g = geo[it.x];
+  const double ih = 1.0 / g.w;
+  for (int k = threadIdx.x; k < it.z; k += 256) {
+    const double4 s = src4[it.y + k];
+    src32[it.y + k] = make_float4((float)((s.x - g.x) * ih), (float)((s.y - g.y) * ih), (float)((s.z - g.z) * ih),
+                                  (float)s.w);
+  }
+}
+
 static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items,
                                                           const double4* __restrict__ geo,
-                                                          const double* __restrict__ surf_uc, int n_uc,
-                                                          int ld_uc_surf, const double4* __restrict__ src4,
+                                                          const float* __restrict__ surf_uc32, int n_uc,
+                                                          int ld_uc_surf, const float4* __restrict__ src32,
                                                           double* __restrict__ chk, int ld_chk) {
   const int4 it = items[blockIdx.x];
   const int j = threadIdx.x, jj = min(j, n_uc - 1);
-  const double4 g = geo[it.x];
-  const double ih = 1.0 / g.w;
-  const float tx = (float)surf_uc[jj], ty = (float)surf_uc[ld_uc_surf + jj], tz = (float)surf_uc[2 * ld_uc_surf + jj];
-  const double4* __restrict__ src = src4 + it.y;
+  const double ih = 1.0 / geo[it.x].w;
+  const float tx = surf_uc32[jj], ty = surf_uc32[ld_uc_surf + jj], tz = surf_uc32[2 * ld_uc_surf + jj];
+  const float4* __restrict__ src = src32 + it.y;
   float acc0 = 0.0f, acc1 = 0.0f;
   int k = 0;
   for (; k + 2 <= it.z; k += 2) {
-    const double4 s0 = src[k], s1 = src[k + 1];
-    const float ax = tx - (float)((s0.x - g.x) * ih), ay = ty - (float)((s0.y - g.y) * ih),
-                az = tz - (float)((s0.z - g.z) * ih);
-    const float bx = tx - (float)((s1.x - g.x) * ih), by = ty - (float)((s1.y - g.y) * ih),
-                bz = tz - (float)((s1.z - g.z) * ih);
-    acc0 = __builtin_fmaf((float)s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-    acc1 = __builtin_fmaf((float)s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+    const float4 s0 = src[k], s1 = src[k + 1];
+    const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+    const float bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
+    acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    acc1 = __builtin_fmaf(s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
   }
   if (k < it.z) {
-    const double4 s0 = src[k];
-    const float ax = tx - (float)((s0.x - g.x) * ih), ay = ty - (float)((s0.y - g.y) * ih),
-                az = tz - (float)((s0.z - g.z) * ih);
-    acc0 = __builtin_fmaf((float)s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    const float4 s0 = src[k];
+    const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+    acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
   }
   if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
 }
 
 static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items,
                                                          const double4* __restrict__ geo,
-                                                         const double* __restrict__ surf_de, int ld_surf, int n_e,
+                                                         const float* __restrict__ surf_de, int ld_surf, int n_e,
                                                          const double* __restrict__ dden, int ld_e,
                                                          const double4* __restrict__ trg4, double* __restrict__ pot) {
   const int4 it = items[blockIdx.x];
@@ -465,14 +477,14 @@ static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict_
   float acc0 = 0.0f, acc1 = 0.0f;
   int j = 0;
   for (; j + 2 <= n_e; j += 2) {
-    const float ax = tx - (float)surf_de[j], ay = ty - (float)surf_de[ld_surf + j], az = tz - (float)surf_de[2 * ld_surf + j];
-    const float bx = tx - (float)surf_de[j + 1], by = ty - (float)surf_de[ld_surf + j + 1],
-                bz = tz - (float)surf_de[2 * ld_surf + j + 1];
+    const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
+    const float bx = tx - surf_de[j + 1], by = ty - surf_de[ld_surf + j + 1],
+                bz = tz - surf_de[2 * ld_surf + j + 1];
     acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
     acc1 = __builtin_fmaf((float)den[j + 1], __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
   }
   if (j < n_e) {
-    const float ax = tx - (float)surf_de[j], ay = ty - (float)surf_de[ld_surf + j], az = tz - (float)surf_de[2 * ld_surf + j];
+    const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
     acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
   }
   if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((double)acc0 + (double)acc1);
@@ -663,6 +675,24 @@ inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, i
 /* Grid geometry for accuracy NP: N = 2 NP points per axis, H = N/2 + 1
  * half-spectrum length along x, F = N*N*H complex frequencies, the surface
  * sits at grid offset NP/2 (fmm/trans.cc:395-417). */
+/* Storage of the V-list spectra: double2 on the FP64 path, float2 on the
+ * mixed path (FP32 complex M2L).  R is the arithmetic type of the Hadamard
+ * multiply-accumulate; the DFTs compute in FP64 and round on store. */
+template <typename ST>
+struct Spec;
+template <>
+struct Spec<double2> {
+  typedef double R;
+  __device__ static double2 mk(double a, double b) { return make_double2(a, b); }
+};
+template <>
+struct Spec<float2> {
+  typedef float R;
+  __device__ static float2 mk(double a, double b) { return make_float2((float)a, (float)b); }
+};
+__device__ __forceinline__ double fm(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fm(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
 template <int NP>
 struct Grid {
   static constexpr int N = 2 * NP, H = NP + 1, F = N * N * H, OFF = NP / 2;
@@ -674,14 +704,14 @@ struct Grid {
  * at offset NP/2 is non-zero, so the three axis passes are pruned to it)
  * and produce the r2c spectrum [kz][ky][kx<=N/2] (FFTW layout), stored
  * interleaved by octant: X[blk][f][oct]. */
-template <int NP>
+template <int NP, typename ST = double2>
 __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks, /* first child node per block */
                                                  const int* __restrict__ xslot,  /* X slot per block */
                                                  const int* __restrict__ depth,
                                                  const double* __restrict__ uden, int ld_e,
                                                  const double2* __restrict__ tw, /* W^m, m < N */
                                                  const int* __restrict__ cube2surf, /* NP^3: sample id or -1 */
-                                                 double2* __restrict__ X) {
+                                                 ST* __restrict__ X) {
   using G = Grid<NP>;
   constexpr int N = G::N, H = G::H, F = G::F, NE = G::NE, OFF = G::OFF;
   __shared__ double gs[8][NE];
@@ -728,7 +758,7 @@ __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks,
   }
   __syncthreads();
   /* z pass, written coalesced as X[slot][f][oct] */
-  double2* out = X + (size_t)xslot[blockIdx.x] * F * 8;
+  ST* out = X + (size_t)xslot[blockIdx.x] * F * 8;
   for (int e = tid; e < 8 * F; e += blockDim.x) {
     const int o = e & 7, f = e >> 3;
     const int kx = f % H, ky = (f / H) % N, kz = f / (H * N);
@@ -740,7 +770,7 @@ __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks,
       re = __builtin_fma(b.x, t.x, __builtin_fma(-b.y, t.y, re));
       im = __builtin_fma(b.x, t.y, __builtin_fma(b.y, t.x, im));
     }
-    out[e] = make_double2(re, im);
+    out[e] = Spec<ST>::mk(re, im);
   }
 }
 
@@ -749,8 +779,8 @@ __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks,
  * tt[g] = 1/(4 pi |2 i - RP_g|) on the wrapped (2NP)^3 grid, full r2c.
  * Output layout T[f][id] so that a wave reads one frequency's 343 entries
  * as uniform (scalar) loads. */
-template <int NP>
-__global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, double2* __restrict__ T) {
+template <int NP, typename ST = double2>
+__global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, ST* __restrict__ T) {
   using G = Grid<NP>;
   constexpr int N = G::N, H = G::H, F = G::F;
   __shared__ double g[N * N * N];
@@ -761,7 +791,7 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, d
   const int i1 = id % 7 - 3, i2 = (id / 7) % 7 - 3, i3 = id / 49 - 3;
   const bool valid = abs(i1) > 1 || abs(i2) > 1 || abs(i3) > 1;
   if (!valid) {
-    for (int f = tid; f < F; f += blockDim.x) T[(size_t)f * 343 + id] = make_double2(0.0, 0.0);
+    for (int f = tid; f < F; f += blockDim.x) T[(size_t)f * 343 + id] = Spec<ST>::mk(0.0, 0.0);
     return;
   }
   if (tid < N) w[tid] = tw[tid];
@@ -806,7 +836,7 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, d
       re = __builtin_fma(v.x, t.x, __builtin_fma(-v.y, t.y, re));
       im = __builtin_fma(v.x, t.y, __builtin_fma(v.y, t.x, im));
     }
-    T[(size_t)e * 343 + id] = make_double2(re, im);
+    T[(size_t)e * 343 + id] = Spec<ST>::mk(re, im);
   }
 }
 
@@ -827,18 +857,20 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, d
  * for an aligned 4x4x4 group of parents instead of 64 x 26 reads from L2);
  * wave h computes target children s in [4h, 4h+4). */
 constexpr int HAD_CAP = 232; /* LDS slots per group */
-template <int NP, int HALF>
-__device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsigned char (*sloc)[64], int lane,
-                                              const double2* __restrict__ Tf, double yr[4], double yi[4]) {
+template <int NP, int HALF, typename ST>
+__device__ __forceinline__ void hadamard_half(const ST (*Xs)[9], const unsigned char (*sloc)[64], int lane,
+                                              const ST* __restrict__ Tf, typename Spec<ST>::R yr[4],
+                                              typename Spec<ST>::R yi[4]) {
+  typedef typename Spec<ST>::R R;
 #pragma unroll 1
   for (int o = 0; o < 27; o++) {
     if (o == 13) continue; /* the parent itself: its children are all adjacent */
     const int ox = o / 9 - 1, oy = (o / 3) % 3 - 1, oz = o % 3 - 1;
-    const double2* __restrict__ To = Tf + (2 * ox + 3) + 7 * (2 * oy + 3) + 49 * (2 * oz + 3);
-    const double2* xs = Xs[sloc[o][lane]];
-    double xr[8], xi[8];
+    const ST* __restrict__ To = Tf + (2 * ox + 3) + 7 * (2 * oy + 3) + 49 * (2 * oz + 3);
+    const ST* xs = Xs[sloc[o][lane]];
+    R xr[8], xi[8];
 #pragma unroll
-    for (int c = 0; c < 8; c++) { const double2 v = xs[c]; xr[c] = v.x; xi[c] = v.y; }
+    for (int c = 0; c < 8; c++) { const ST v = xs[c]; xr[c] = v.x; xi[c] = v.y; }
 #pragma unroll
     for (int c = 0; c < 8; c++) {
 #pragma unroll
@@ -847,9 +879,9 @@ __device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsi
         const int ex = ((c >> 2) & 1) - ((sidx >> 2) & 1);
         const int ey = ((c >> 1) & 1) - ((sidx >> 1) & 1);
         const int ez = (c & 1) - (sidx & 1);
-        const double2 t = To[ex + 7 * ey + 49 * ez];
-        yr[q] = __builtin_fma(t.x, xr[c], __builtin_fma(-t.y, xi[c], yr[q]));
-        yi[q] = __builtin_fma(t.x, xi[c], __builtin_fma(t.y, xr[c], yi[q]));
+        const ST t = To[ex + 7 * ey + 49 * ez];
+        yr[q] = fm(t.x, xr[c], fm(-t.y, xi[c], yr[q]));
+        yi[q] = fm(t.x, xi[c], fm(t.y, xr[c], yi[q]));
       }
     }
   }
@@ -861,19 +893,20 @@ __device__ __forceinline__ void hadamard_half(const double2 (*Xs)[9], const unsi
  * loaded into registers while the current one is multiplied, and the
  * parents' 27 union indices are staged once per block. */
 constexpr int HAD_FB = 8;
-template <int NP>
+template <int NP, typename ST = double2>
 __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ groups, /* {first parent, count, union beg, union size} */
                                                   const int* __restrict__ uslots,  /* X slots of each union */
                                                   const unsigned char* __restrict__ loc, /* 27 union indices per parent */
                                                   const int* __restrict__ yslot,   /* Y slot per parent */
-                                                  const double2* __restrict__ T,
-                                                  const double2* __restrict__ X,
-                                                  double2* __restrict__ Y) {
+                                                  const ST* __restrict__ T,
+                                                  const ST* __restrict__ X,
+                                                  ST* __restrict__ Y) {
+  typedef typename Spec<ST>::R R;
   using G = Grid<NP>;
   constexpr int F = G::F;
   static_assert(F % HAD_FB == 0, "frequency blocks");
   constexpr int PER = (HAD_CAP * 8 + 127) / 128;
-  __shared__ double2 Xs[HAD_CAP][9];
+  __shared__ ST Xs[HAD_CAP][9];
   __shared__ unsigned char sloc[27][64];
   const int4 g = groups[blockIdx.x];
   const int f0 = blockIdx.y * HAD_FB;
@@ -886,12 +919,12 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
     const int e = tid + 128 * k;
     us[k] = e < ne8 ? uslots[g.z + (e >> 3)] : 0;
   }
-  double2 pre[PER];
+  ST pre[PER];
   auto fetch = [&](int f) {
 #pragma unroll
     for (int k = 0; k < PER; k++) {
       const int e = tid + 128 * k;
-      pre[k] = e < ne8 ? X[((size_t)us[k] * F + f) * 8 + (e & 7)] : make_double2(0.0, 0.0);
+      pre[k] = e < ne8 ? X[((size_t)us[k] * F + f) * 8 + (e & 7)] : Spec<ST>::mk(0.0, 0.0);
     }
   };
   fetch(f0);
@@ -913,16 +946,16 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
     }
     __syncthreads();
     if (fi + 1 < HAD_FB) fetch(f + 1);
-    const double2* __restrict__ Tf = T + (size_t)f * 343;
-    double yr[4] = {0.0, 0.0, 0.0, 0.0}, yi[4] = {0.0, 0.0, 0.0, 0.0};
+    const ST* __restrict__ Tf = T + (size_t)f * 343;
+    R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
     if (half == 0)
-      hadamard_half<NP, 0>(Xs, sloc, lane, Tf, yr, yi);
+      hadamard_half<NP, 0, ST>(Xs, sloc, lane, Tf, yr, yi);
     else
-      hadamard_half<NP, 1>(Xs, sloc, lane, Tf, yr, yi);
+      hadamard_half<NP, 1, ST>(Xs, sloc, lane, Tf, yr, yi);
     if (lane < g.y) {
-      double2* yb = Y + ((size_t)ys * F + f) * 8 + 4 * half;
+      ST* yb = Y + ((size_t)ys * F + f) * 8 + 4 * half;
 #pragma unroll
-      for (int q = 0; q < 4; q++) yb[q] = make_double2(yr[q], yi[q]);
+      for (int q = 0; q < 4; q++) yb[q] = Spec<ST>::mk(yr[q], yi[q]);
     }
   }
 }
@@ -931,9 +964,9 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
  * the NP^3 cube holding the check surface; normalised by 1/N^3 and added
  * into the downward check values (compute_ifft_trg, fmm/omp_sse_block.cc:482-533:
  * FFTW c2r semantics, the kx = 0 and kx = N/2 imaginary parts are ignored). */
-template <int NP>
+template <int NP, typename ST = double2>
 __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks, const int* __restrict__ yslot,
-                                                 const double2* __restrict__ Y, const double2* __restrict__ tw,
+                                                 const ST* __restrict__ Y, const double2* __restrict__ tw,
                                                  const int* __restrict__ surf2cube, /* NE: (ix*NP+iy)*NP+iz */
                                                  double* __restrict__ dchk, int ld_e) {
   using G = Grid<NP>;
@@ -945,7 +978,7 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
   const int first = blocks[blockIdx.x];
   if (tid < N) w[tid] = tw[tid];
   __syncthreads();
-  const double2* in = Y + (size_t)yslot[blockIdx.x] * F * 8;
+  const ST* in = Y + (size_t)yslot[blockIdx.x] * F * 8;
   /* z pass: thread per (ky, kx, oct), NP outputs in registers */
   for (int e = tid; e < N * H * 8; e += blockDim.x) {
     const int o = e & 7, kk = e >> 3; /* kk = ky*H + kx */
@@ -953,7 +986,8 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
 #pragma unroll
     for (int iz = 0; iz < NP; iz++) { re[iz] = 0.0; im[iz] = 0.0; }
     for (int kz = 0; kz < N; kz++) {
-      const double2 v = in[(size_t)(kz * N * H + kk) * 8 + o];
+      const ST vs = in[(size_t)(kz * N * H + kk) * 8 + o];
+      const double2 v = make_double2(vs.x, vs.y);
 #pragma unroll
       for (int iz = 0; iz < NP; iz++) {
         const double2 t = w[(kz * (iz + OFF)) % N]; /* conj applied below */

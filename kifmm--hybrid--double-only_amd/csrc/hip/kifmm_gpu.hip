@@ -68,6 +68,10 @@ struct kifmm_gpu {
   double *UC2UE = nullptr, *UE2UC = nullptr, *DC2DE = nullptr, *DE2DC = nullptr;
   double *surf_ue = nullptr, *surf_uc = nullptr, *surf_de = nullptr, *surf_dc = nullptr;
   double2 *X = nullptr, *Y = nullptr, *T = nullptr, *tw = nullptr;
+  /* mixed path: FP32 spectra, FP32 unit surfaces, FP32 leaf-local sources */
+  float2 *Xf = nullptr, *Yf = nullptr, *Tf = nullptr;
+  float *surf_uc32 = nullptr, *surf_de32 = nullptr;
+  float4* src32 = nullptr;
   int *cube2surf = nullptr, *surf2cube = nullptr;
   int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *tnum = nullptr, *depth = nullptr;
   int *wptr = nullptr, *widx = nullptr, *xptr = nullptr, *xidx = nullptr;
@@ -541,8 +545,16 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     host_surface(np, 1.0, s, ls); TRY(dev_upload(g, &g->surf_ue, s.data(), s.size()));
     std::fill(s.begin(), s.end(), 0.0);
     host_surface(np + 2, 3.0, s, ls); TRY(dev_upload(g, &g->surf_uc, s.data(), s.size()));
+    if (g->precision == KIFMM_PREC_MIXED) {
+      std::vector<float> f(s.begin(), s.end());
+      TRY(dev_upload(g, &g->surf_uc32, f.data(), f.size()));
+    }
     std::fill(s.begin(), s.end(), 0.0);
     host_surface(np, 3.0, s, ls); TRY(dev_upload(g, &g->surf_de, s.data(), s.size()));
+    if (g->precision == KIFMM_PREC_MIXED) {
+      std::vector<float> f(s.begin(), s.end());
+      TRY(dev_upload(g, &g->surf_de32, f.data(), f.size()));
+    }
     std::fill(s.begin(), s.end(), 0.0);
     host_surface(np, 1.0, s, ls); TRY(dev_upload(g, &g->surf_dc, s.data(), s.size()));
     const int N = g->N;
@@ -592,9 +604,16 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   TRY(dev_upload(g, &g->hloc, P.hloc.data(), P.hloc.size()));
   TRY(dev_upload(g, &g->hslot, P.hslot.data(), P.hslot.size()));
   const size_t spec = (size_t)(P.nblk + 1) * g->F * 8;
-  TRY(dev_zero(g, &g->X, spec));
-  TRY(dev_zero(g, &g->Y, spec));
-  TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
+  if (g->precision == KIFMM_PREC_MIXED) {
+    TRY(dev_zero(g, &g->Xf, spec));
+    TRY(dev_zero(g, &g->Yf, spec));
+    TRY(dev_zero(g, &g->Tf, (size_t)g->F * 343));
+    TRY(dev_zero(g, &g->src32, (size_t)std::max(g->n_src, 1)));
+  } else {
+    TRY(dev_zero(g, &g->X, spec));
+    TRY(dev_zero(g, &g->Y, spec));
+    TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
+  }
   g->n_wleaves = (int)P.wleaves.size();
   g->n_xnodes = (int)P.xnodes.size();
   g->n_l2t = (int)P.l2t_items.size();
@@ -642,26 +661,32 @@ int fill(kifmm_gpu* g, double* p, size_t n) {
   return KIFMM_OK;
 }
 
-template <int NP>
-int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
+template <int NP, typename ST>
+int vlist_phase(kifmm_gpu* g, ST* X, ST* Y, ST* T, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   if (g->n_vblocks > 0)
-    hipLaunchKernelGGL(k_dft_fwd<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->depth,
-                       g->uden, g->ld_e, g->tw, g->cube2surf, g->X);
+    hipLaunchKernelGGL((k_dft_fwd<NP, ST>), dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot,
+                       g->depth, g->uden, g->ld_e, g->tw, g->cube2surf, X);
   LAUNCH_CHECK();
   REC(e1);
-  hipLaunchKernelGGL(k_trans<NP>, dim3(343), dim3(256), 0, g->stream, g->tw, g->T);
+  hipLaunchKernelGGL((k_trans<NP, ST>), dim3(343), dim3(256), 0, g->stream, g->tw, T);
   LAUNCH_CHECK();
   REC(e2);
   if (g->n_hgroups > 0)
-    hipLaunchKernelGGL(k_hadamard<NP>, dim3(g->n_hgroups, g->F / HAD_FB), dim3(128), 0, g->stream, g->hgroups, g->huslots,
-                       g->hloc, g->hslot, g->T, g->X, g->Y);
+    hipLaunchKernelGGL((k_hadamard<NP, ST>), dim3(g->n_hgroups, g->F / HAD_FB), dim3(128), 0, g->stream, g->hgroups,
+                       g->huslots, g->hloc, g->hslot, T, X, Y);
   LAUNCH_CHECK();
   REC(e3);
   if (g->n_vblocks > 0)
-    hipLaunchKernelGGL(k_dft_inv<NP>, dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, g->Y,
+    hipLaunchKernelGGL((k_dft_inv<NP, ST>), dim3(g->n_vblocks), dim3(512), 0, g->stream, g->vblocks, g->vslot, Y,
                        g->tw, g->surf2cube, g->dchk, g->ld_e);
   LAUNCH_CHECK();
   return KIFMM_OK;
+}
+
+template <int NP>
+int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
+  if (g->precision == KIFMM_PREC_MIXED) return vlist_phase<NP, float2>(g, g->Xf, g->Yf, g->Tf, e1, e2, e3);
+  return vlist_phase<NP, double2>(g, g->X, g->Y, g->T, e1, e2, e3);
 }
 
 template <bool MASK>
@@ -708,10 +733,12 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
-    if (g->precision == KIFMM_PREC_MIXED)
+    if (g->precision == KIFMM_PREC_MIXED) {
+      hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
+                         g->src32);
       hipLaunchKernelGGL(k_s2m_mixed, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
-                         g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
-    else
+                         g->geo, g->surf_uc32, g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
+    } else
       hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
                          g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
     LAUNCH_CHECK();
@@ -751,8 +778,14 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
     TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
-    hipLaunchKernelGGL(g->precision == KIFMM_PREC_MIXED ? k_l2t_mixed : k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream,
-                       g->l2t_items, g->geo, g->surf_de, g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
+  {
+    if (g->precision == KIFMM_PREC_MIXED)
+      hipLaunchKernelGGL(k_l2t_mixed, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de32,
+                         g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
+    else
+      hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de, g->ld_surf,
+                         g->n_e, g->dden, lde, g->trg4, g->pot);
+  }
   LAUNCH_CHECK();
   REC(ev[9]);
   return KIFMM_OK;
