@@ -181,12 +181,20 @@ class CostModel:
                          d.get("meta", {}))
 
 
-def phase_times(problem: K.Problem, precision: int, steps: int = 3, warmup: int = 1, graph: bool = False):
-    """Measured mean phase times (ms) and wall ms per evaluation."""
+def phase_times(problem: K.Problem, precision: int, steps: int = 3, warmup: int = 1, graph: bool = False,
+                slow_ms: float = 50.0):
+    """Measured mean phase times (ms) and wall ms per evaluation.  An
+    evaluation slower than slow_ms is measured once (its first run)."""
     ev = K.GpuEvaluator(problem, precision=precision, graph=graph)
-    for _ in range(warmup):
-        ev.evaluate(copy=False)
     acc = np.zeros(len(PHASES))
+    t0 = time.perf_counter()
+    _, t = ev.evaluate(copy=False)
+    first = (time.perf_counter() - t0) * 1e3
+    if first > slow_ms:
+        ev.close()
+        return np.array([t[p] for p in PHASES]), first
+    for _ in range(max(warmup - 1, 0)):
+        ev.evaluate(copy=False)
     t0 = time.perf_counter()
     for _ in range(steps):
         _, t = ev.evaluate(copy=False)

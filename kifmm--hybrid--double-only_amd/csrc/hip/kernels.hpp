@@ -1064,9 +1064,12 @@ __device__ __forceinline__ void apply_tile(const double4* tile, int ns, const do
   }
 }
 
-/* W-list (wlist_eval__gpu, fmm/cuda.cu:2070-2197): one block per target leaf. */
+/* W-list (wlist_eval__gpu, fmm/cuda.cu:2070-2197): item = {target leaf,
+ * first target, target count <= WX_CHUNK}; a leaf with many targets is
+ * spread over several blocks. */
+constexpr int WX_CHUNK = 256;
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_wlist(const int* __restrict__ leaves, const int* __restrict__ wptr,
+__global__ __launch_bounds__(256) void k_wlist(const int4* __restrict__ items, const int* __restrict__ wptr,
                                                const int* __restrict__ widx, const int* __restrict__ child,
                                                const int* __restrict__ snum, const int* __restrict__ sbeg,
                                                const int* __restrict__ tbeg, const int* __restrict__ tnum,
@@ -1075,7 +1078,8 @@ __global__ __launch_bounds__(256) void k_wlist(const int* __restrict__ leaves, c
                                                const double4* __restrict__ src4, const double4* __restrict__ trg4,
                                                double* __restrict__ pot) {
   __shared__ double4 tile[1024];
-  const int t = leaves[blockIdx.x];
+  const int4 it = items[blockIdx.x];
+  const int t = it.x, tb = tbeg[t] + it.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   for (int k = wptr[t]; k < wptr[t + 1]; k++) {
     const int s = widx[k];
@@ -1086,14 +1090,17 @@ __global__ __launch_bounds__(256) void k_wlist(const int* __restrict__ leaves, c
       const int ns = direct ? stage_sources(tile, 1024, src4, sbeg[s] + base, n - base, nullptr, 0, geo[s], nullptr, tid, 256)
                             : stage_sources(tile, 1024, nullptr, 0, n, surf_ue, ld_surf, geo[s], uden + (size_t)s * ld_e, tid, 256);
       __syncthreads();
-      apply_tile<MASK>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+      apply_tile<MASK>(tile, ns, trg4, tb, it.z, nullptr, 0, geo[t], pot + tb, lane, wave, 4);
     }
   }
 }
 
-/* X-list (xlist_eval__gpu, fmm/cuda.cu:2380-2493): one block per target node. */
+/* X-list (xlist_eval__gpu, fmm/cuda.cu:2380-2493): item = {target node,
+ * first target, target count <= WX_CHUNK, direct}; direct items (a leaf
+ * with fewer than n_e targets) evaluate at their targets, the others at the
+ * node's DC check surface (one item per node). */
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_xlist(const int* __restrict__ nodes, const int* __restrict__ xptr,
+__global__ __launch_bounds__(256) void k_xlist(const int4* __restrict__ items, const int* __restrict__ xptr,
                                                const int* __restrict__ xidx, const int* __restrict__ child,
                                                const int* __restrict__ snum, const int* __restrict__ sbeg,
                                                const int* __restrict__ tbeg, const int* __restrict__ tnum,
@@ -1102,9 +1109,10 @@ __global__ __launch_bounds__(256) void k_xlist(const int* __restrict__ nodes, co
                                                const double4* __restrict__ src4, const double4* __restrict__ trg4,
                                                double* __restrict__ pot) {
   __shared__ double4 tile[1024];
-  const int t = nodes[blockIdx.x];
+  const int4 it = items[blockIdx.x];
+  const int t = it.x, tb = tbeg[t] + it.y;
   const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool direct = child[t] == -1 && tnum[t] < n_e;
+  const bool direct = it.w != 0;
   for (int k = xptr[t]; k < xptr[t + 1]; k++) {
     const int s = xidx[k];
     for (int base = 0; base < snum[s]; base += 1024) {
@@ -1112,7 +1120,7 @@ __global__ __launch_bounds__(256) void k_xlist(const int* __restrict__ nodes, co
       const int ns = stage_sources(tile, 1024, src4, sbeg[s] + base, snum[s] - base, nullptr, 0, geo[s], nullptr, tid, 256);
       __syncthreads();
       if (direct)
-        apply_tile<MASK>(tile, ns, trg4, tbeg[t], tnum[t], nullptr, 0, geo[t], pot + tbeg[t], lane, wave, 4);
+        apply_tile<MASK>(tile, ns, trg4, tb, it.z, nullptr, 0, geo[t], pot + tb, lane, wave, 4);
       else
         apply_tile<false>(tile, ns, nullptr, 0, n_e, surf_dc, ld_surf, geo[t], dchk + (size_t)t * ld_e, lane, wave, 4);
     }

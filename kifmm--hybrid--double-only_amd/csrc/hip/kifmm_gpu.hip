@@ -86,8 +86,8 @@ struct kifmm_gpu {
   int4* hgroups = nullptr; int n_hgroups = 0;
   int* huslots = nullptr;
   unsigned char* hloc = nullptr;
-  int* wleaves = nullptr; int n_wleaves = 0;
-  int* xnodes = nullptr; int n_xnodes = 0;
+  int4* witems = nullptr; int n_witems = 0;
+  int4* xitems = nullptr; int n_xitems = 0;
   int4* l2t_items = nullptr; int n_l2t = 0;
   size_t chk_cols = 0;
   double* ws = nullptr;
@@ -220,6 +220,7 @@ struct Plan {
   size_t chk_cols = 0;
   size_t ws_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
+  std::vector<int4> witems, xitems;
   std::vector<int4> l2t_items;
   std::vector<int4> hgroups;
   std::vector<int> huslots;
@@ -448,6 +449,22 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
   }
   /* apply_tile clamps target indices to [tbeg, tbeg + tnum - 1]: the leaves
    * handed to k_wlist / k_xlist (direct) / k_l2t must own a target */
+  /* W / X items: target chunks of at most WX_CHUNK per block */
+  for (int i : P.wleaves)
+    for (int c = 0; c < t->trg_num[i]; c += WX_CHUNK)
+      P.witems.push_back(make_int4(i, c, std::min(WX_CHUNK, t->trg_num[i] - c), 0));
+  for (int i : P.xnodes) {
+    if (t->child[i] == -1 && t->trg_num[i] < ne) {
+      for (int c = 0; c < t->trg_num[i]; c += WX_CHUNK)
+        P.xitems.push_back(make_int4(i, c, std::min(WX_CHUNK, t->trg_num[i] - c), 1));
+    } else {
+      P.xitems.push_back(make_int4(i, 0, 0, 0));
+    }
+  }
+  for (const int4& it : P.witems)
+    if (it.z < 1 || it.y < 0 || it.y + it.z > t->trg_num[it.x]) return fail("W-list item out of range");
+  for (const int4& it : P.xitems)
+    if (it.w && (it.z < 1 || it.y < 0 || it.y + it.z > t->trg_num[it.x])) return fail("X-list item out of range");
   for (int i : P.wleaves)
     if (t->trg_num[i] < 1) return fail("W-list leaf without targets");
   for (int i : P.xnodes)
@@ -614,11 +631,11 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     TRY(dev_zero(g, &g->Y, spec));
     TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
   }
-  g->n_wleaves = (int)P.wleaves.size();
-  g->n_xnodes = (int)P.xnodes.size();
+  g->n_witems = (int)P.witems.size();
+  g->n_xitems = (int)P.xitems.size();
   g->n_l2t = (int)P.l2t_items.size();
-  TRY(dev_upload(g, &g->wleaves, P.wleaves.data(), P.wleaves.size()));
-  TRY(dev_upload(g, &g->xnodes, P.xnodes.data(), P.xnodes.size()));
+  TRY(dev_upload(g, &g->witems, P.witems.data(), P.witems.size()));
+  TRY(dev_upload(g, &g->xitems, P.xitems.data(), P.xitems.size()));
   TRY(dev_upload(g, &g->l2t_items, P.l2t_items.data(), P.l2t_items.size()));
   return KIFMM_OK;
 }
@@ -706,14 +723,14 @@ int point_phases_u(kifmm_gpu* g) {
 
 template <bool MASK>
 int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
-  if ((mask & KIFMM_PHASE_W) && g->n_wleaves > 0)
-    hipLaunchKernelGGL(k_wlist<MASK>, dim3(g->n_wleaves), dim3(256), 0, g->stream, g->wleaves, g->wptr, g->widx,
+  if ((mask & KIFMM_PHASE_W) && g->n_witems > 0)
+    hipLaunchKernelGGL(k_wlist<MASK>, dim3(g->n_witems), dim3(256), 0, g->stream, g->witems, g->wptr, g->widx,
                        g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_ue, g->ld_surf, g->n_e, g->uden,
                        g->ld_e, g->src4, g->trg4, g->pot);
   LAUNCH_CHECK();
   REC(ew);
-  if ((mask & KIFMM_PHASE_X) && g->n_xnodes > 0)
-    hipLaunchKernelGGL(k_xlist<MASK>, dim3(g->n_xnodes), dim3(256), 0, g->stream, g->xnodes, g->xptr, g->xidx,
+  if ((mask & KIFMM_PHASE_X) && g->n_xitems > 0)
+    hipLaunchKernelGGL(k_xlist<MASK>, dim3(g->n_xitems), dim3(256), 0, g->stream, g->xitems, g->xptr, g->xidx,
                        g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_dc, g->ld_surf, g->n_e, g->dchk,
                        g->ld_e, g->src4, g->trg4, g->pot);
   LAUNCH_CHECK();

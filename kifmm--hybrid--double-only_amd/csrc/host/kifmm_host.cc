@@ -112,13 +112,23 @@ void draw_ellipse(Rand48& r, int n, double cx, double cy, double cz, double ax, 
   }
 }
 
-/* 8 centres uniform in [0.2,0.8]^3, per point a cluster draw and two
+/* Mixture of 8 isotropic Gaussians, sigma 0.02 in [0,1]^3: centres uniform
+ * in [0.2,0.8]^3, drawn once per problem (targets, when separate, are a
+ * second draw from the same mixture); per point a cluster draw and two
  * Box-Muller pairs (the fourth normal is discarded); clipped to [0,1],
  * mapped by x' = 2x - 1. */
-void draw_gaussmix(Rand48& r, int n, double* x, double* y, double* z) {
-  double cen[8][3];
+struct Centres {
+  double c[8][3];
+};
+
+Centres draw_centres(Rand48& r) {
+  Centres k;
   for (int c = 0; c < 8; c++)
-    for (int d = 0; d < 3; d++) cen[c][d] = 0.2 + 0.6 * r.next();
+    for (int d = 0; d < 3; d++) k.c[c][d] = 0.2 + 0.6 * r.next();
+  return k;
+}
+
+void draw_gaussmix(Rand48& r, const Centres& k, int n, double* x, double* y, double* z) {
   for (int i = 0; i < n; i++) {
     int c = (int)(r.next() * 8.0);
     if (c > 7) c = 7;
@@ -131,7 +141,7 @@ void draw_gaussmix(Rand48& r, int n, double* x, double* y, double* z) {
     }
     double v[3];
     for (int d = 0; d < 3; d++) {
-      double t = cen[c][d] + 0.02 * g[d];
+      double t = k.c[c][d] + 0.02 * g[d];
       t = t < 0.0 ? 0.0 : (t > 1.0 ? 1.0 : t);
       v[d] = 2.0 * t - 1.0;
     }
@@ -160,9 +170,10 @@ bool generate(kifmm_problem* p, const char* dist, long seed, int coincide) {
     if (separate)
       draw_ellipse(p->rng, n, 0.0, 0.0, 0.0, 1.0, 1.0, 0.25, p->tx.data(), p->ty.data(), p->tz.data());
   } else if (!strcmp(dist, "gaussmix")) {
-    draw_gaussmix(p->rng, n, p->sx.data(), p->sy.data(), p->sz.data());
+    const Centres k = draw_centres(p->rng);
+    draw_gaussmix(p->rng, k, n, p->sx.data(), p->sy.data(), p->sz.data());
     separate = !coincide;
-    if (separate) draw_gaussmix(p->rng, n, p->tx.data(), p->ty.data(), p->tz.data());
+    if (separate) draw_gaussmix(p->rng, k, n, p->tx.data(), p->ty.data(), p->tz.data());
   } else {
     return false;
   }

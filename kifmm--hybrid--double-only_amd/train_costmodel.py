@@ -14,7 +14,10 @@ sweep and against a fixed q = 64 on the FP64 path.
     python train_costmodel.py --out DIR [--train 64] [--held 16]
 
 writes DIR/costmodel_weights.json, DIR/costmodel_data.json and
-DIR/costmodel_report.json and prints one line per input.
+DIR/costmodel_report.json and prints one line per measured configuration.
+The collection can be split: --collect A:B measures training inputs
+A..B-1 only (writing DIR/costmodel_data.json), and --data F1 F2 ... trains
+on collected files instead of measuring.
 """
 from __future__ import annotations
 
@@ -44,7 +47,7 @@ def draw_inputs(count: int, seed0: int, rng_seed: int):
     return out
 
 
-def sweep(inp, qs, steps):
+def sweep(inp, qs, steps, tag=""):
     """Every (q, path) of one input: features and measured phase times."""
     n, dist, np_, seed = inp
     base = K.Problem.create(n, dist, 64, np_=np_, seed=seed)
@@ -59,6 +62,7 @@ def sweep(inp, qs, steps):
             ph, wall = CM.phase_times(prob, prec, steps=steps)
             rows.append({"n": n, "dist": dist, "np": np_, "seed": seed, "q": q, "prec": prec,
                          "features": [float(x) for x in f], "phases_ms": [float(x) for x in ph], "wall_ms": wall})
+            print(f"  {tag} q={q} prec={prec}: {wall:.2f} ms", flush=True)
         del prob
     return rows, pts
 
@@ -71,19 +75,30 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--epochs", type=int, default=4000)
     ap.add_argument("--lr", type=float, default=3e-3)
+    ap.add_argument("--collect", default=None, help="A:B -- only measure training inputs A..B-1")
+    ap.add_argument("--data", nargs="*", default=None, help="train on these collected files")
     args = ap.parse_args()
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
     qs = CM.Q_CHOICES
     t0 = time.time()
     data = []
-    for k, inp in enumerate(draw_inputs(args.train, 1000, 7)):
-        rows, _ = sweep(inp, qs, args.steps)
-        data += rows
-        best = min(rows, key=lambda r: r["wall_ms"])
-        print(f"train {k + 1}/{args.train} n={inp[0]} {inp[1]} NP={inp[2]}: best q={best['q']} "
-              f"prec={best['prec']} {best['wall_ms']:.2f} ms  [{time.time() - t0:.0f} s]", flush=True)
-    (out / "costmodel_data.json").write_text(json.dumps(data))
+    if args.data:
+        for f in args.data:
+            data += json.loads(Path(f).read_text())
+    else:
+        inputs = draw_inputs(args.train, 1000, 7)
+        a, b = (0, args.train) if args.collect is None else map(int, args.collect.split(":"))
+        for k in range(a, b):
+            inp = inputs[k]
+            rows, _ = sweep(inp, qs, args.steps, f"train {k + 1}")
+            data += rows
+            best = min(rows, key=lambda r: r["wall_ms"])
+            print(f"train {k + 1}/{args.train} n={inp[0]} {inp[1]} NP={inp[2]}: best q={best['q']} "
+                  f"prec={best['prec']} {best['wall_ms']:.2f} ms  [{time.time() - t0:.0f} s]", flush=True)
+        (out / "costmodel_data.json").write_text(json.dumps(data))
+        if args.collect is not None:
+            return 0
     X = np.array([r["features"] for r in data])
     Y = np.array([r["phases_ms"] for r in data])
     model = CM.CostModel.fit(X, Y, epochs=args.epochs, lr=args.lr)
@@ -94,7 +109,7 @@ def main():
     tuner = CM.Tuner(model, qs)
     report = []
     for k, inp in enumerate(draw_inputs(args.held, 5000, 11)):
-        rows, pts = sweep(inp, qs, args.steps)
+        rows, pts = sweep(inp, qs, args.steps, f"held {k + 1}")
         by = {(r["q"], r["prec"]): r for r in rows}
         best = min(rows, key=lambda r: r["wall_ms"])
         q, prec, pred, _ = tuner.choose(pts, inp[2])

@@ -629,13 +629,16 @@ void setup_surfaces(oracle_problem* p) {
   p->n_uc = p->SP[UC].n;
 }
 
-/* gaussmix centres and draws: 8 centres uniform in [0.2,0.8]^3, then per
- * point a cluster index and a Box-Muller pair per two coordinates; clipped
- * to [0,1] and mapped by x' = 2x - 1.  (No reference counterpart.) */
-void draw_gaussmix(unsigned short* st, int N, double* x, double* y, double* z) {
-  double cen[8][3];
+/* gaussmix: 8 centres uniform in [0.2,0.8]^3 (drawn once; a separate
+ * target set is a second draw from the same mixture), then per point a
+ * cluster index and a Box-Muller pair per two coordinates; clipped to
+ * [0,1] and mapped by x' = 2x - 1.  (No reference counterpart.) */
+void draw_centres(unsigned short* st, double cen[8][3]) {
   for (int c = 0; c < 8; c++)
     for (int d = 0; d < 3; d++) cen[c][d] = 0.2 + 0.6 * erand48(st);
+}
+
+void draw_gaussmix(unsigned short* st, const double cen[8][3], int N, double* x, double* y, double* z) {
   const double sigma = 0.02;
   for (int i = 0; i < N; i++) {
     int c = (int)(erand48(st) * 8.0);
@@ -712,8 +715,10 @@ bool generate(oracle_problem* p, const char* dist, long seed, int coincide) {
     }
     sep_targets = !coincide;
   } else if (d == "gaussmix") {
-    draw_gaussmix(st, N, p->sx.data(), p->sy.data(), p->sz.data());
-    if (!coincide) draw_gaussmix(st, N, p->tx.data(), p->ty.data(), p->tz.data());
+    double cen[8][3];
+    draw_centres(st, cen);
+    draw_gaussmix(st, cen, N, p->sx.data(), p->sy.data(), p->sz.data());
+    if (!coincide) draw_gaussmix(st, cen, N, p->tx.data(), p->ty.data(), p->tz.data());
     sep_targets = !coincide;
   } else {
     return false;
