@@ -32,15 +32,16 @@ constexpr double FAR = 1.0e8; /* coordinate of padding sources */
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 
-/* x^(-1/2): hardware estimate refined by one third-order step of the
- * Newton-Raphson family for x^(-1/2):  e = 1 - x y^2,
- * y <- y (1 + e/2 + 3e^2/8), which cubes the relative error. */
+/* x^(-1/2): the hardware estimate (v_rsq_f64, relative error below 2^-24,
+ * measured max 5.2e-8) refined by one Newton step, e = 1 - x y^2,
+ * y <- y + y e / 2, which leaves a relative error of (3/2) e0^2 <= 2e-14
+ * (4e-15 for the measured estimate): far below the 1e-12 parity bar, and
+ * one instruction per pair cheaper than a third-order step. */
 __device__ __forceinline__ double rsqrt_fp64(double x) {
   double y = __builtin_amdgcn_rsq(x);
   double h = x * y;
   double e = __builtin_fma(-h, y, 1.0);
-  double p = __builtin_fma(e, 0.375, 0.5);
-  return __builtin_fma(y * e, p, y);
+  return __builtin_fma(y * e, 0.5, y);
 }
 
 template <bool MASK>
