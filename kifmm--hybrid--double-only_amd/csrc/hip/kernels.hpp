@@ -376,21 +376,22 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
 template <bool MASK>
 __device__ __forceinline__ double sum_uniform(double tx, double ty, double tz, const double4* __restrict__ src,
                                               int n) {
-  double acc0 = 0.0, acc1 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int k = 0;
-  for (; k + 2 <= n; k += 2) {
-    const double4 s0 = src[k], s1 = src[k + 1];
-    const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    const double bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
-    acc0 = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
-    acc1 = __builtin_fma(s1.w, inv_r<MASK>(__builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx))), acc1);
+  for (; k + 4 <= n; k += 4) { /* four sources per step: one batch of scalar loads */
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double4 s0 = src[k + u];
+      const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+      acc[u] = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
+    }
   }
-  if (k < n) {
+  for (; k < n; k++) {
     const double4 s0 = src[k];
     const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    acc0 = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+    acc[0] = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
   }
-  return acc0 + acc1;
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 /* S2M check potentials (up_eval__gpu, fmm/cuda.cu:108-190 first half): one
@@ -1137,27 +1138,30 @@ static __global__ __launch_bounds__(64) void k_l2t(const int4* __restrict__ item
                                                    const double* __restrict__ surf_de, int ld_surf, int n_e,
                                                    const double* __restrict__ dden, int ld_e,
                                                    const double4* __restrict__ trg4, double* __restrict__ pot) {
+  /* in leaf units (t - c) / h the DE surface is the same unit-box surface
+   * for every leaf, read as wave-uniform values; the sum is scaled by 1/h
+   * (h a power of two: exact) */
   const int4 it = items[blockIdx.x];
   const int k = threadIdx.x;
   const double4 q = trg4[it.y + min(k, it.z - 1)];
   const double4 g = geo[it.x];
+  const double ih = 1.0 / g.w;
+  const double tx = (q.x - g.x) * ih, ty = (q.y - g.y) * ih, tz = (q.z - g.z) * ih;
   const double* __restrict__ den = dden + (size_t)it.x * ld_e;
-  double acc0 = 0.0, acc1 = 0.0;
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int j = 0;
-  for (; j + 2 <= n_e; j += 2) {
-    const double ax = q.x - (g.x + g.w * surf_de[j]), ay = q.y - (g.y + g.w * surf_de[ld_surf + j]),
-                 az = q.z - (g.z + g.w * surf_de[2 * ld_surf + j]);
-    const double bx = q.x - (g.x + g.w * surf_de[j + 1]), by = q.y - (g.y + g.w * surf_de[ld_surf + j + 1]),
-                 bz = q.z - (g.z + g.w * surf_de[2 * ld_surf + j + 1]);
-    acc0 = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
-    acc1 = __builtin_fma(den[j + 1], inv_r<false>(__builtin_fma(bz, bz, __builtin_fma(by, by, bx * bx))), acc1);
+  for (; j + 4 <= n_e; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double ax = tx - surf_de[j + u], ay = ty - surf_de[ld_surf + j + u], az = tz - surf_de[2 * ld_surf + j + u];
+      acc[u] = __builtin_fma(den[j + u], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
+    }
   }
-  if (j < n_e) {
-    const double ax = q.x - (g.x + g.w * surf_de[j]), ay = q.y - (g.y + g.w * surf_de[ld_surf + j]),
-                 az = q.z - (g.z + g.w * surf_de[2 * ld_surf + j]);
-    acc0 = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc0);
+  for (; j < n_e; j++) {
+    const double ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
+    acc[0] = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
   }
-  if (k < it.z) pot[it.y + k] += OOFP_R * (acc0 + acc1);
+  if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
 /* Sampled direct sum for validation (validate, fmm/driver.cc:141-206, on

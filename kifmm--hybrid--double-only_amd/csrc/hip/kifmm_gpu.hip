@@ -27,6 +27,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "kifmm_hip.h"
@@ -93,6 +94,8 @@ struct kifmm_gpu {
   double* ws = nullptr;
   size_t ws_len = 0;
   kifmm_work_counts counts{};
+  /* pinned host staging of the potentials (device -> host copy at PCIe rate) */
+  double* pinned = nullptr;
   /* HIP graph of the whole launch sequence (KIFMM_GPU_GRAPH) */
   int use_graph = 0;
   bool capturing = false;
@@ -835,9 +838,23 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     TRY(enqueue(g, mask));
   }
   if (pot_leaf)
-    HIP_TRY(hipMemcpyAsync(pot_leaf, g->pot, sizeof(double) * g->n_trg, hipMemcpyDeviceToHost, g->stream));
+    HIP_TRY(hipMemcpyAsync(g->pinned ? g->pinned : pot_leaf, g->pot, sizeof(double) * g->n_trg,
+                           hipMemcpyDeviceToHost, g->stream));
   HIP_TRY(hipEventRecord(ev[10], g->stream));
   HIP_TRY(hipEventSynchronize(ev[10]));
+  double host_copy_ms = 0.0;
+  if (pot_leaf && g->pinned) { /* pinned staging -> caller's buffer, a few host threads */
+    const double t0 = kifmm::wall_seconds();
+    const size_t n = (size_t)g->n_trg;
+    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / (1 << 16)));
+    std::vector<std::thread> th;
+    for (int k = 0; k < nt; k++) {
+      const size_t b = n * k / nt, e = n * (k + 1) / nt;
+      th.emplace_back([=] { std::memcpy(pot_leaf + b, g->pinned + b, sizeof(double) * (e - b)); });
+    }
+    for (auto& x : th) x.join();
+    host_copy_ms = (kifmm::wall_seconds() - t0) * 1e3;
+  }
   if (times) {
     float ms[11] = {0};
     for (int i = 1; i <= 10; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
@@ -855,12 +872,12 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     times->wlist = ms[7];
     times->xlist = ms[8];
     times->down = ms[9];
-    times->copy = ms[10];
-    times->total = tot;
+    times->copy = ms[10] + host_copy_ms;
+    times->total = tot + host_copy_ms;
     if (graph) { /* one graph launch: no per-phase events inside it */
       std::memset(times, 0, sizeof *times);
-      times->copy = ms[10];
-      times->total = tot;
+      times->copy = ms[10] + host_copy_ms;
+      times->total = tot + host_copy_ms;
     }
   }
   return KIFMM_OK;
@@ -941,6 +958,10 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
     kifmm_gpu_destroy(g);
     return r;
   }
+  /* pinned staging for the potentials; without it the copy goes straight
+   * to the caller's (pageable) buffer */
+  if (hipHostMalloc((void**)&g->pinned, sizeof(double) * std::max(g->n_trg, 1), hipHostMallocDefault) != hipSuccess)
+    g->pinned = nullptr;
   if (hipDeviceSynchronize() != hipSuccess) {
     set_error("device synchronisation failed after upload");
     kifmm_gpu_destroy(g);
@@ -970,6 +991,7 @@ int kifmm_gpu_destroy(kifmm_gpu* g) {
   if (!g) return KIFMM_OK;
   (void)hipSetDevice(g->dev);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
+  if (g->pinned) (void)hipHostFree(g->pinned);
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   for (void* p : g->allocs) (void)hipFree(p);
