@@ -127,6 +127,19 @@ int kifmm_gpu_direct(kifmm_gpu* g, int nt, const double* tx, const double* ty, c
 double kifmm_problem_validate_gpu(kifmm_problem* p, kifmm_gpu* g, const double* pot_orig, int nv, int* chk_ids,
                                   double* direct_out);
 
+/* kifmm_problem_create / kifmm_problem_from_points with the octree, the
+ * U/V/W/X lists and the gather permutations built on GPU `device`.  The
+ * result is identical, array for array, to the host build, which follows
+ * the preprocessing of fmm/input.cc:203-460 (preproc, fmm/evaluate.h:18-20);
+ * only kifmm_problem_stats' tree time differs.  Fails (NULL,
+ * kifmm_last_error) on invalid input, on a device error, or when the tree
+ * would need more than 30 levels (coincident points beyond pts_max). */
+kifmm_problem* kifmm_problem_create_gpu(int device, int n, const char* distribution, int pts_max,
+                                        const kifmm_problem_opts* opts);
+kifmm_problem* kifmm_problem_from_points_gpu(int device, int n, const double* sx, const double* sy, const double* sz,
+                                             const double* den, const double* tx, const double* ty, const double* tz,
+                                             int pts_max, int np);
+
 #ifdef __cplusplus
 }
 #endif

@@ -212,7 +212,11 @@ void split_range(int* perm, int* tmp, int b, int e, const double* x, const doubl
   std::memcpy(perm + b, out.data(), sizeof(int) * (e - b));
 }
 
-void build_tree(kifmm_problem* p) {
+/* deepest level a tree may reach: box paths are ints and box centres
+ * -1 + (2 path + 1) 2^-depth stay exact doubles well within it */
+constexpr int MAX_LEVELS = 30;
+
+bool build_tree(kifmm_problem* p) {
   const int n = p->n;
   std::vector<int>& par = p->parent;
   std::vector<int>& chd = p->child;
@@ -232,6 +236,10 @@ void build_tree(kifmm_problem* p) {
     int next = end;
     for (int k = beg; k < end; k++)
       if (re[k] - rb[k] > p->q) { first[k - beg] = next; next += 8; }
+    if (next > end && p->level.size() >= (size_t)MAX_LEVELS) {
+      set_error("tree build: more than 30 levels (too many coincident points for the points per box)");
+      return false;
+    }
     par.resize(next); chd.resize(next); dep.resize(next); pth.resize(3 * (size_t)next);
     rb.resize(next); re.resize(next);
 #pragma omp parallel for schedule(dynamic, 16)
@@ -302,6 +310,7 @@ void build_tree(kifmm_problem* p) {
     }
   }
   p->tperm.swap(tgath);
+  return true;
 }
 
 /* ------------------------------------------------------------ lists */
@@ -409,7 +418,7 @@ void gather(kifmm_problem* p) {
 
 bool finish(kifmm_problem* p) {
   double t0 = wall_seconds();
-  build_tree(p);
+  if (!build_tree(p)) return false;
   build_lists(p);
   gather(p);
   p->t_tree = wall_seconds() - t0;
@@ -575,7 +584,10 @@ void build_operators(kifmm_problem* p) {
 
 kifmm_problem* checked_create(kifmm_problem* p) {
   try {
-    finish(p);
+    if (!finish(p)) {
+      delete p;
+      return nullptr;
+    }
   } catch (const std::bad_alloc&) {
     set_error("host allocation failed while building the tree");
     delete p;
@@ -610,6 +622,22 @@ long kifmm_getenv_seed(void) {
 
 kifmm_problem* kifmm_problem_create(int n, const char* distribution, int pts_max,
                                     const kifmm_problem_opts* opts) {
+  kifmm_problem* p = kifmm::problem_generate(n, distribution, pts_max, opts);
+  return p ? checked_create(p) : nullptr;
+}
+
+kifmm_problem* kifmm_problem_from_points(int n, const double* sx, const double* sy, const double* sz,
+                                         const double* den, const double* tx, const double* ty,
+                                         const double* tz, int pts_max, int np) {
+  kifmm_problem* p = kifmm::problem_points(n, sx, sy, sz, den, tx, ty, tz, pts_max, np);
+  return p ? checked_create(p) : nullptr;
+}
+
+}  // extern "C"
+
+/* Points of a new problem without its tree (the generators of
+ * kifmm_problem_create), for builders outside this file. */
+kifmm_problem* kifmm::problem_generate(int n, const char* distribution, int pts_max, const kifmm_problem_opts* opts) {
   if (n <= 0 || pts_max <= 0 || !distribution) {
     set_error("kifmm_problem_create: n and pts_max must be positive");
     return nullptr;
@@ -637,12 +665,11 @@ kifmm_problem* kifmm_problem_create(int n, const char* distribution, int pts_max
     return nullptr;
   }
   p->t_gen = wall_seconds() - t0;
-  return checked_create(p);
+  return p;
 }
 
-kifmm_problem* kifmm_problem_from_points(int n, const double* sx, const double* sy, const double* sz,
-                                         const double* den, const double* tx, const double* ty,
-                                         const double* tz, int pts_max, int np) {
+kifmm_problem* kifmm::problem_points(int n, const double* sx, const double* sy, const double* sz, const double* den,
+                                     const double* tx, const double* ty, const double* tz, int pts_max, int np) {
   if (n <= 0 || pts_max <= 0 || np < 3 || !sx || !sy || !sz || !den || !tx || !ty || !tz) {
     set_error("kifmm_problem_from_points: invalid argument");
     return nullptr;
@@ -663,8 +690,29 @@ kifmm_problem* kifmm_problem_from_points(int n, const double* sx, const double* 
   p->den.assign(den, den + n);
   p->tx.assign(tx, tx + n); p->ty.assign(ty, ty + n); p->tz.assign(tz, tz + n);
   p->coincident = (sx == tx && sy == ty && sz == tz) ? 1 : 0;
-  return checked_create(p);
+  return p;
 }
+
+void kifmm::problem_coords(const kifmm_problem* p, const double* s3[3], const double* t3[3], int* n, int* q) {
+  s3[0] = p->sx.data(); s3[1] = p->sy.data(); s3[2] = p->sz.data();
+  t3[0] = p->tx.data(); t3[1] = p->ty.data(); t3[2] = p->tz.data();
+  *n = p->n;
+  *q = p->q;
+}
+
+void kifmm::problem_install(kifmm_problem* p, TreeArrays&& t, double seconds) {
+  p->level.swap(t.level); p->parent.swap(t.parent); p->child.swap(t.child); p->depth.swap(t.depth);
+  p->tag.swap(t.tag); p->path.swap(t.path); p->sbeg.swap(t.sbeg); p->snum.swap(t.snum);
+  p->tbeg.swap(t.tbeg); p->tnum.swap(t.tnum); p->sperm.swap(t.sperm); p->tperm.swap(t.tperm);
+  for (int k = 0; k < 4; k++) { p->lptr[k].swap(t.lptr[k]); p->lidx[k].swap(t.lidx[k]); }
+  const double t0 = wall_seconds();
+  gather(p);
+  p->t_tree = seconds + (wall_seconds() - t0);
+  p->n_e = p->np * p->np * p->np - (p->np - 2) * (p->np - 2) * (p->np - 2);
+  p->n_uc = (p->np + 2) * (p->np + 2) * (p->np + 2) - p->np * p->np * p->np;
+}
+
+extern "C" {
 
 /* Point files: 8-byte magic "KIFMMPT1", int64 n, int32 has_targets,
  * int32 reserved (0), then n little-endian doubles each of sx, sy, sz, den

@@ -96,6 +96,8 @@ def _sig(name, res, args):
 _sig("kifmm_last_error", C.c_char_p, [])
 _sig("kifmm_problem_create", C.c_void_p, [C.c_int, C.c_char_p, C.c_int, C.POINTER(ProblemOpts)])
 _sig("kifmm_problem_from_points", C.c_void_p, [C.c_int] + [_dp] * 7 + [C.c_int, C.c_int])
+_sig("kifmm_problem_create_gpu", C.c_void_p, [C.c_int, C.c_int, C.c_char_p, C.c_int, C.POINTER(ProblemOpts)])
+_sig("kifmm_problem_from_points_gpu", C.c_void_p, [C.c_int, C.c_int] + [_dp] * 7 + [C.c_int, C.c_int])
 _sig("kifmm_problem_destroy", None, [C.c_void_p])
 _sig("kifmm_problem_view", C.c_int, [C.c_void_p, C.POINTER(TreeView)])
 _sig("kifmm_problem_operators", C.c_int, [C.c_void_p, C.POINTER(Operators)])
@@ -159,22 +161,31 @@ class Problem:
         self._h = handle
 
     @classmethod
-    def create(cls, n, distribution="uniform", pts_max=64, np_=None, seed=None, coincide=None, nthreads=0):
+    def create(cls, n, distribution="uniform", pts_max=64, np_=None, seed=None, coincide=None, nthreads=0,
+               device=None):
+        """device=None builds the tree and lists on the host, an int on that GPU (same arrays)."""
         opts = ProblemOpts(np_ if np_ else -1, -1 if seed is None else seed,
                            -1 if coincide is None else int(coincide), nthreads)
-        h = lib.kifmm_problem_create(n, distribution.encode(), pts_max, C.byref(opts))
+        if device is None:
+            h = lib.kifmm_problem_create(n, distribution.encode(), pts_max, C.byref(opts))
+        else:
+            h = lib.kifmm_problem_create_gpu(int(device), n, distribution.encode(), pts_max, C.byref(opts))
         if not h:
             raise KifmmError(-1, last_error())
         return cls(h)
 
     @classmethod
-    def from_points(cls, sx, sy, sz, den, tx=None, ty=None, tz=None, pts_max=64, np_=6):
+    def from_points(cls, sx, sy, sz, den, tx=None, ty=None, tz=None, pts_max=64, np_=6, device=None):
         arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (sx, sy, sz, den)]
         if tx is None:
             t = arrs[:3]
         else:
             t = [np.ascontiguousarray(a, dtype=np.float64) for a in (tx, ty, tz)]
-        h = lib.kifmm_problem_from_points(len(arrs[0]), *[_dptr(a) for a in arrs + t], pts_max, np_)
+        ptrs = [_dptr(a) for a in arrs + t]
+        if device is None:
+            h = lib.kifmm_problem_from_points(len(arrs[0]), *ptrs, pts_max, np_)
+        else:
+            h = lib.kifmm_problem_from_points_gpu(int(device), len(arrs[0]), *ptrs, pts_max, np_)
         if not h:
             raise KifmmError(-1, last_error())
         p = cls(h)

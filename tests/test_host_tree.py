@@ -200,3 +200,14 @@ def test_point_file_roundtrip(tmp_path):
     h.write_bytes(f.read_bytes()[:1000])
     with pytest.raises(K.KifmmError):
         K.Problem.from_file(h, pts_max=16, np_=4)
+
+
+def test_too_many_coincident_points_is_an_error():
+    """More than pts_max coincident sources can never be separated: the build
+    stops at 30 levels with an error instead of splitting forever; exactly
+    pts_max coincident sources stay in one leaf."""
+    pts = np.zeros((3, 50)) + 0.3
+    with pytest.raises(K.KifmmError, match="30 levels"):
+        K.Problem.from_points(*pts, np.ones(50), pts_max=8, np_=4)
+    t = K.Problem.from_points(*pts, np.ones(50), pts_max=50, np_=4).tree()
+    assert t["n_nodes"] == 1 and t["src_num"][0] == 50
