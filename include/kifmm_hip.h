@@ -26,15 +26,20 @@ typedef struct kifmm_gpu kifmm_gpu;
 
 enum {
   KIFMM_PREC_FP64 = 0, /* plain FP64 path */
-  KIFMM_PREC_MIXED = 1 /* NP <= 4: FP32 Gram-form distances on MFMA with FP64 recompute of
-                          close pairs, FP32 rsqrt, FP64 accumulation; FP32 S2M / L2T;
-                          FP64 pseudo-inverse applies and V-list */
+  KIFMM_PREC_MIXED = 1 /* NP <= 4: U-list distances in FP32 Gram form on MFMA, FP32 rsqrt,
+                          FP64 terms for pairs closer than 1/8 of the target leaf's
+                          half-width, FP64 accumulation; FP32 S2M / L2T and V-list
+                          spectra; FP64 pseudo-inverse applies, M2M / L2L and W / X */
 };
 
 enum {
-  KIFMM_GPU_GRAPH = 1 /* capture the whole evaluation in one HIP graph at the first full
-                         evaluate, replay it afterwards (phase times then report only
-                         total and copy) */
+  KIFMM_GPU_GRAPH = 1, /* capture the whole evaluation in one HIP graph at the first full
+                          evaluate, replay it afterwards (phase times then report only
+                          total and copy) */
+  KIFMM_GPU_MIXED_INLINE = 2 /* mixed path: test for close pairs inside the U-list loop
+                                instead of listing them once at create time (the list is
+                                skipped by itself when it would exceed 16 pairs per
+                                target on average) */
 };
 
 typedef struct kifmm_gpu_options {
@@ -74,7 +79,10 @@ typedef struct kifmm_work_counts {
   double gemm_flops;   /* 2*m*k per applied operator column         */
   double fft_bytes;    /* HBM bytes of the forward + inverse DFT kernels */
   double hadamard_bytes; /* compulsory HBM bytes of the Hadamard kernel */
-  double reserved[5];
+  double close_pairs;  /* mixed path: U-list pairs closer than 1/8 leaf half-width, listed
+                          at create time; -1 when they are tested in the loop instead;
+                          0 on the FP64 path */
+  double reserved[4];
 } kifmm_work_counts;
 
 int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops,
@@ -131,7 +139,7 @@ double kifmm_problem_validate_gpu(kifmm_problem* p, kifmm_gpu* g, const double* 
  * U/V/W/X lists and the gather permutations built on GPU `device`.  The
  * result is identical, array for array, to the host build, which follows
  * the preprocessing of fmm/input.cc:203-460 (preproc, fmm/evaluate.h:18-20);
- * only kifmm_problem_stats' tree time differs.  Fails (NULL,
+ * only kifmm_problem_setup_times' tree time differs.  Fails (NULL,
  * kifmm_last_error) on invalid input, on a device error, or when the tree
  * would need more than 30 levels (coincident points beyond pts_max). */
 kifmm_problem* kifmm_problem_create_gpu(int device, int n, const char* distribution, int pts_max,

@@ -33,6 +33,9 @@ int kifmm_test_gemm(int M, int K, const double* A, int ncols, const double* B, c
 /* y = x^(-1/2) as the point kernels compute it; ymask is the r == 0-masked
  * form (0 where x == 0). */
 int kifmm_test_rsqrt(int n, const double* x, double* y, double* ymask);
+/* y = min(1, 1/sqrt|x|) in FP32 as the mixed U-list forms it (v_rsq_f32 with
+ * the |x| and clamp modifiers) */
+int kifmm_test_rsq_clamp_f32(int n, const float* x, float* y);
 
 /* Forward transform of nblocks groups of 8 equivalent densities
  * (den[(8b+o)*n_e + j], scaled by 2^depth[8b+o] on the grid): spectra are

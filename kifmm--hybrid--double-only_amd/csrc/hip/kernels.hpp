@@ -37,19 +37,29 @@ typedef double v4d __attribute__((ext_vector_type(4)));
  * y <- y + y e / 2, which leaves a relative error of (3/2) e0^2 <= 2e-14
  * (4e-15 for the measured estimate): far below the 1e-12 parity bar, and
  * one instruction per pair cheaper than a third-order step. */
-__device__ __forceinline__ double rsqrt_fp64(double x) {
-  double y = __builtin_amdgcn_rsq(x);
-  double h = x * y;
-  double e = __builtin_fma(-h, y, 1.0);
-  return __builtin_fma(y * e, 0.5, y);
+/* 2 / sqrt(x): the v_rsq_f64 estimate y (relative error <= 2^-24.2 on this
+ * chip) and one Newton step written as y (3 - x y^2) = (2 / sqrt(x)) (1 -
+ * 3e^2/2 + ...): three VALU instructions after the rsq.  The factor 2 is
+ * left in; pair sums built from it are halved once at the end (an exact
+ * power-of-two scale), which saves a multiply per pair. */
+__device__ __forceinline__ double rsqrt2_fp64(double x) {
+  const double y = __builtin_amdgcn_rsq(x);
+  return y * __builtin_fma(-x, y * y, 3.0);
 }
+__device__ __forceinline__ double rsqrt_fp64(double x) { return 0.5 * rsqrt2_fp64(x); }
 
+/* 2 / r, and 1 / r; MASK: r = 0 (coincident points) gives 0 */
 template <bool MASK>
-__device__ __forceinline__ double inv_r(double r2) {
-  double y = rsqrt_fp64(r2);
+__device__ __forceinline__ double inv_r2(double r2) {
+  double y = rsqrt2_fp64(r2);
   if (MASK) y = (r2 > 0.0) ? y : 0.0;
   return y;
 }
+template <bool MASK>
+__device__ __forceinline__ double inv_r(double r2) {
+  return 0.5 * inv_r2<MASK>(r2);
+}
+constexpr double HALF_OOFP_R = 0.5 * OOFP_R; /* scales sums of 2 w / r */
 
 /* 8 partial sums per lane -> lane l holds the wave total of target (l >> 3)
  * in lanes with (l & 7) == 0.  Fixed order, deterministic. */
@@ -81,7 +91,7 @@ __device__ __forceinline__ double wave_reduce8(const double acc[8], int lane) {
   return c;
 }
 
-/* acc[t] += sum over LDS sources [0, ns) (ns multiple of 64) of w / |x_t - s| */
+/* acc[t] += sum over LDS sources [0, ns) (ns multiple of 64) of 2 w / |x_t - s| */
 template <bool MASK>
 __device__ __forceinline__ void eval8_lds(const double4* __restrict__ src, int ns, const double tx[8],
                                           const double ty[8], const double tz[8], double acc[8],
@@ -92,7 +102,7 @@ __device__ __forceinline__ void eval8_lds(const double4* __restrict__ src, int n
     for (int t = 0; t < 8; t++) {
       const double dx = tx[t] - p.x, dy = ty[t] - p.y, dz = tz[t] - p.z;
       const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-      acc[t] = __builtin_fma(p.w, inv_r<MASK>(r2), acc[t]);
+      acc[t] = __builtin_fma(p.w, inv_r2<MASK>(r2), acc[t]);
     }
   }
 }
@@ -194,7 +204,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
     if (grp < ngroups) {
       const double sum = wave_reduce8(acc[k], lane);
       const int t = 8 * grp + (lane >> 3);
-      if ((lane & 7) == 0 && t < it.z) pot[it.y + t] = OOFP_R * sum;
+      if ((lane & 7) == 0 && t < it.z) pot[it.y + t] = HALF_OOFP_R * sum;
     }
   }
 }
@@ -365,6 +375,217 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
   }
 }
 
+/* Mixed-precision U-list with the close pairs listed ahead (NP <= 4).
+ * Coordinates are centred on the target leaf and scaled by 1 / (tau h)
+ * with tau = MIX_CLOSE = 1/8 (a power of two: exact), so a pair is close,
+ * r < tau h, exactly when its scaled distance r' is below 1.  The Gram
+ * MFMA gives r'^2 as in k_p2p_ulist_mixed; every pair then adds
+ * w * min(1, 1 / r') in FP32, the min and a sign guard being the free
+ * clamp and |x| modifiers of v_rsq_f32: no per-pair test.  The pairs with
+ * r' < 1 in FP64 were found once per plan (k_close_pairs: per target, a
+ * CSR list of source indices); when a target's sum is written, each adds
+ * w (1 / r' - 1) in FP64 (w (0 - 1) for r = 0 with MASK), which replaces
+ * the clamped w by the FP64 term.  Pairs near r' = 1 whose FP32 and FP64
+ * sides disagree are off by the FP32 error of a pair at distance tau h
+ * (~1e-4 relative to that pair, as for any FP32 pair). */
+constexpr double MIX_CLOSE = 0.125;
+
+template <int G>
+__device__ __forceinline__ void mixed_tau_chunks(const float4* __restrict__ sB, const float* __restrict__ sW,
+                                                 int nchunks, const float* a, const v4f* c, double (*acc)[4],
+                                                 int lane) {
+  const int j = lane & 15, k4 = lane >> 4;
+  const float* sBf = (const float*)sB;
+  float part[G][4];
+#pragma unroll
+  for (int q = 0; q < G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) part[q][e] = 0.0f;
+  for (int ch = 0; ch < nchunks; ch += 2) { /* nchunks is even */
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int s = (ch + u) * 16 + j;
+      const float b = sBf[4 * s + k4];
+      const float w = sW[s];
+#pragma unroll
+      for (int q = 0; q < G; q++) {
+        const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b, c[q], 0, 0, 0);
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+          part[q][e] = __builtin_fmaf(
+              w, __builtin_amdgcn_fmed3f(__builtin_amdgcn_rsqf(__builtin_fabsf(d[e])), 0.0f, 1.0f), part[q][e]);
+      }
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[q][e] += (double)part[q][e];
+}
+
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                             const int* __restrict__ rng_ptr,
+                                                             const int2* __restrict__ rng,
+                                                             const double4* __restrict__ geo,
+                                                             const double4* __restrict__ src4,
+                                                             const double4* __restrict__ trg4,
+                                                             const int* __restrict__ cptr,
+                                                             const int* __restrict__ cidx,
+                                                             double* __restrict__ pot) {
+  __shared__ float4 sB[P2P_TILE];
+  __shared__ float sW[P2P_TILE];
+  __shared__ double4 sT[P2P_MAXT];
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const int4 it = items[item];
+  const double4 g = geo[it.w];
+  const double sc = (1.0 / MIX_CLOSE) / g.w; /* 1 / (tau h), a power of two */
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ngroups = (it.z + 15) >> 4;
+  /* scaled targets; slots past the count repeat the last target */
+  for (int k = tid; k < ngroups * 16; k += 256) {
+    const double4 q = trg4[it.y + min(k, it.z - 1)];
+    const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
+    sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+  }
+  __syncthreads();
+  float a[2];
+  v4f c[2];
+  double acc[2][4];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int grp = min(wave + 4 * q, ngroups - 1);
+    const double4 t = sT[16 * grp + (lane & 15)];
+    const int k4 = lane >> 4;
+    a[q] = (float)(k4 == 0 ? -2.0 * t.x : k4 == 1 ? -2.0 * t.y : k4 == 2 ? -2.0 * t.z : 1.0);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
+      acc[q][e] = 0.0;
+    }
+  }
+  int ri = rng_ptr[it.x], ro = 0;
+  const int rlast = rng_ptr[it.x + 1];
+  while (ri < rlast) {
+    __syncthreads();
+    int fill = 0;
+    while (fill < P2P_TILE && ri < rlast) {
+      const int2 r = rng[ri];
+      const int take = min(r.y - ro, P2P_TILE - fill);
+      for (int k = tid; k < take; k += 256) {
+        const double4 q = src4[r.x + ro + k];
+        const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
+        sB[fill + k] = make_float4((float)x, (float)y, (float)z, (float)__builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+        sW[fill + k] = (float)q.w;
+      }
+      fill += take;
+      ro += take;
+      if (ro == r.y) {
+        ri++;
+        ro = 0;
+      }
+    }
+    const int ns = (fill + 31) & ~31; /* whole chunk pairs (P2P_TILE is a multiple of 32) */
+    for (int k = fill + tid; k < ns; k += 256) {
+      sB[k] = make_float4((float)MIX_FAR, (float)MIX_FAR, (float)MIX_FAR, (float)(3.0 * MIX_FAR * MIX_FAR));
+      sW[k] = 0.0f;
+    }
+    __syncthreads();
+    if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, ns >> 4, a, c, acc, lane);
+    else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, ns >> 4, a, c, acc, lane);
+  }
+  const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int grp = wave + 4 * q;
+    if (grp < ngroups) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        double v = acc[q][e];
+        v += __shfl_xor(v, 8, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 1, 64);
+        const int t = 16 * grp + 4 * (lane >> 4) + e;
+        if ((lane & 15) == 0 && t < it.z) {
+          const int tg = it.y + t;
+          const double4 tp = sT[t];
+          double corr = 0.0;
+          for (int k = cptr[tg]; k < cptr[tg + 1]; k++) {
+            const double4 sp = src4[cidx[k]];
+            const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
+            const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+            corr = __builtin_fma(sp.w, inv_r<MASK>(r2) - 1.0, corr);
+          }
+          pot[tg] = scale * (v + corr);
+        }
+      }
+    }
+  }
+}
+
+/* The close pairs of the mixed U-list (geometry only, found once per
+ * plan): for each target of an item, the sources of its U-list ranges at
+ * FP64 scaled distance r' < 1 (r < MIX_CLOSE h), in range order.
+ * FILL = false writes the count per target; FILL = true writes the source
+ * indices from the exclusive offsets cptr. */
+template <bool FILL>
+__global__ __launch_bounds__(128) void k_close_pairs(const int4* __restrict__ items, const int* __restrict__ rng_ptr,
+                                                     const int2* __restrict__ rng, const double4* __restrict__ geo,
+                                                     const double4* __restrict__ src4,
+                                                     const double4* __restrict__ trg4, int* __restrict__ cnt,
+                                                     const int* __restrict__ cptr, int* __restrict__ cidx) {
+  __shared__ double4 tile[P2P_TILE];
+  const int4 it = items[blockIdx.x];
+  const double4 g = geo[it.w];
+  const double sc = (1.0 / MIX_CLOSE) / g.w;
+  const int k = threadIdx.x;
+  const bool live = k < it.z;
+  double tx = 0.0, ty = 0.0, tz = 0.0;
+  if (live) {
+    const double4 q = trg4[it.y + k];
+    tx = (q.x - g.x) * sc; ty = (q.y - g.y) * sc; tz = (q.z - g.z) * sc;
+  }
+  int n = 0, pos = 0, end = 0;
+  if (FILL && live) {
+    pos = cptr[it.y + k];
+    end = cptr[it.y + k + 1];
+  }
+  int ri = rng_ptr[it.x], ro = 0;
+  const int rlast = rng_ptr[it.x + 1];
+  while (ri < rlast) {
+    __syncthreads();
+    int fill = 0;
+    while (fill < P2P_TILE && ri < rlast) {
+      const int2 r = rng[ri];
+      const int take = min(r.y - ro, P2P_TILE - fill);
+      for (int m = k; m < take; m += 128) {
+        const double4 q = src4[r.x + ro + m];
+        tile[fill + m] = make_double4((q.x - g.x) * sc, (q.y - g.y) * sc, (q.z - g.z) * sc, (double)(r.x + ro + m));
+      }
+      fill += take;
+      ro += take;
+      if (ro == r.y) {
+        ri++;
+        ro = 0;
+      }
+    }
+    __syncthreads();
+    if (live)
+      for (int m = 0; m < fill; m++) {
+        const double4 sp = tile[m];
+        const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
+        if (__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) < 1.0) {
+          if (FILL && pos + n < end) cidx[pos + n] = (int)sp.w; /* the count pass sized it */
+          n++;
+        }
+      }
+  }
+  if (!FILL && live) cnt[it.y + k] = n;
+}
+
 /* ------------------------------------------------------------------ S2M */
 
 /* Lanes as targets, sources wave-uniform.  Every lane of the wave walks the
@@ -372,7 +593,8 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
  * and feeds to the VALU as SGPR operands: no LDS, no barrier, and no
  * cross-lane reduction (each lane owns its sum).  Used where the sources of
  * a whole wave are one short list: a leaf's points (S2M) or a box's
- * equivalent surface (L2T).  Fixed summation order, deterministic. */
+ * equivalent surface (L2T).  Returns the sum of 2 w / r, in a fixed
+ * summation order (deterministic). */
 template <bool MASK>
 __device__ __forceinline__ double sum_uniform(double tx, double ty, double tz, const double4* __restrict__ src,
                                               int n) {
@@ -383,13 +605,13 @@ __device__ __forceinline__ double sum_uniform(double tx, double ty, double tz, c
     for (int u = 0; u < 4; u++) {
       const double4 s0 = src[k + u];
       const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-      acc[u] = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
+      acc[u] = __builtin_fma(s0.w, inv_r2<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
     }
   }
   for (; k < n; k++) {
     const double4 s0 = src[k];
     const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    acc[0] = __builtin_fma(s0.w, inv_r<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
+    acc[0] = __builtin_fma(s0.w, inv_r2<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
   }
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
@@ -411,7 +633,7 @@ static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ ite
   const double ty = g.y + g.w * surf_uc[ld_uc_surf + jj];
   const double tz = g.z + g.w * surf_uc[2 * ld_uc_surf + jj];
   const double acc = sum_uniform<false>(tx, ty, tz, src4 + it.y, it.z);
-  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * acc;
+  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = HALF_OOFP_R * acc;
 }
 
 /* Mixed-precision S2M and L2T (NP <= 4): the same lane-per-target scheme
@@ -1062,7 +1284,7 @@ __device__ __forceinline__ void apply_tile(const double4* tile, int ns, const do
     eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
     const double sum = wave_reduce8(acc, lane);
     const int j = grp * 8 + (lane >> 3);
-    if ((lane & 7) == 0 && j < ntrg) out[j] += OOFP_R * sum;
+    if ((lane & 7) == 0 && j < ntrg) out[j] += HALF_OOFP_R * sum;
   }
 }
 
@@ -1154,14 +1376,14 @@ static __global__ __launch_bounds__(64) void k_l2t(const int4* __restrict__ item
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const double ax = tx - surf_de[j + u], ay = ty - surf_de[ld_surf + j + u], az = tz - surf_de[2 * ld_surf + j + u];
-      acc[u] = __builtin_fma(den[j + u], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
+      acc[u] = __builtin_fma(den[j + u], inv_r2<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
     }
   }
   for (; j < n_e; j++) {
     const double ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-    acc[0] = __builtin_fma(den[j], inv_r<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
+    acc[0] = __builtin_fma(den[j], inv_r2<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
   }
-  if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+  if (k < it.z) pot[it.y + k] += HALF_OOFP_R * ih * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
 /* Sampled direct sum for validation (validate, fmm/driver.cc:141-206, on

@@ -79,6 +79,10 @@ struct kifmm_gpu {
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int* rng_ptr = nullptr; int2* rng = nullptr;
+  /* mixed path: per target, the U-list sources closer than MIX_CLOSE h */
+  int* cptr = nullptr; int* cidx = nullptr;
+  int close_listed = 0;
+  long long n_close = 0;
   int4* s2m_items = nullptr; int n_s2m = 0;
   GemmList s2m_gemm;
   std::vector<LevelOps> lv;
@@ -643,6 +647,41 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   return KIFMM_OK;
 }
 
+/* Close pairs of the mixed U-list (k_close_pairs): counts, offsets on the
+ * host, then the indices.  An input whose list would exceed 16 pairs per
+ * target on average keeps the in-loop test (k_p2p_ulist_mixed). */
+int close_list(kifmm_gpu* g) {
+  int* cnt = nullptr;
+  TRY(dev_zero(g, &cnt, (size_t)std::max(g->n_trg, 1)));
+  if (g->n_p2p > 0)
+    hipLaunchKernelGGL(k_close_pairs<false>, dim3(g->n_p2p), dim3(128), 0, g->stream, g->p2p_items, g->rng_ptr,
+                       g->rng, g->geo, g->src4, g->trg4, cnt, nullptr, nullptr);
+  HIP_TRY(hipGetLastError());
+  std::vector<int> h((size_t)std::max(g->n_trg, 1));
+  HIP_TRY(hipMemcpyAsync(h.data(), cnt, sizeof(int) * h.size(), hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const long long cap = std::max(16LL * g->n_trg, 1LL << 20);
+  std::vector<int> ptr((size_t)g->n_trg + 1, 0);
+  long long tot = 0;
+  for (int i = 0; i < g->n_trg; i++) {
+    if (tot > cap) return KIFMM_OK; /* close_listed stays 0 */
+    ptr[i] = (int)tot;
+    tot += h[i];
+  }
+  if (tot > cap) return KIFMM_OK;
+  ptr[g->n_trg] = (int)tot;
+  TRY(dev_upload(g, &g->cptr, ptr.data(), ptr.size()));
+  TRY(dev_zero(g, &g->cidx, (size_t)std::max<long long>(tot, 1)));
+  if (g->n_p2p > 0)
+    hipLaunchKernelGGL(k_close_pairs<true>, dim3(g->n_p2p), dim3(128), 0, g->stream, g->p2p_items, g->rng_ptr,
+                       g->rng, g->geo, g->src4, g->trg4, nullptr, g->cptr, g->cidx);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  g->n_close = tot;
+  g->close_listed = 1;
+  return KIFMM_OK;
+}
+
 #define LAUNCH_CHECK()                                                           \
   do {                                                                           \
     hipError_t e_ = hipGetLastError();                                           \
@@ -713,7 +752,11 @@ template <bool MASK>
 int point_phases_u(kifmm_gpu* g) {
   if (g->n_p2p > 0) {
     const int per_xcd = (g->n_p2p + 7) / 8;
-    if (g->precision == KIFMM_PREC_MIXED)
+    if (g->precision == KIFMM_PREC_MIXED && g->close_listed)
+      hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
+                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx,
+                         g->pot);
+    else if (g->precision == KIFMM_PREC_MIXED)
       hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
                          g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
     else
@@ -950,6 +993,8 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
   }
   try {
     r = build(g, tree, ops);
+    if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED && !(opts && (opts->flags & KIFMM_GPU_MIXED_INLINE)))
+      r = close_list(g);
   } catch (const std::bad_alloc&) {
     set_error("host allocation failed in kifmm_gpu_create");
     r = KIFMM_ENOMEM;
@@ -1078,6 +1123,7 @@ int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifm
 int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out) {
   if (!g || !out) return KIFMM_EINVAL;
   *out = g->counts;
+  out->close_pairs = g->precision != KIFMM_PREC_MIXED ? 0.0 : g->close_listed ? (double)g->n_close : -1.0;
   return KIFMM_OK;
 }
 

@@ -77,6 +77,12 @@ __global__ void k_rsqrt_probe(const double* x, double* y, double* ymask, int n) 
   }
 }
 
+/* the clamped reciprocal square root of the mixed U-list: min(1, 1/sqrt|x|) */
+__global__ void k_rsq_clamp_probe(const float* x, float* y, int n) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = __builtin_amdgcn_fmed3f(__builtin_amdgcn_rsqf(__builtin_fabsf(x[i])), 0.0f, 1.0f);
+}
+
 constexpr int rup(int x, int m) { return (x + m - 1) / m * m; }
 
 template <int NP>
@@ -293,6 +299,21 @@ int kifmm_test_rsqrt(int n, const double* x, double* y, double* ymask) {
   r = down(y, dy, n);
   if (r != KIFMM_OK) return r;
   return down(ymask, dm, n);
+}
+
+int kifmm_test_rsq_clamp_f32(int n, const float* x, float* y) {
+  if (n <= 0 || !x || !y) return KIFMM_EINVAL;
+  DevBufs B;
+  float* dx = B.up(x, n);
+  float* dy = B.up<float>(nullptr, n);
+  if (!dx || !dy) {
+    set_error("device allocation failed");
+    return KIFMM_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_rsq_clamp_probe, dim3((n + 255) / 256), dim3(256), 0, 0, dx, dy, n);
+  const int r = finish("k_rsq_clamp_probe");
+  if (r != KIFMM_OK) return r;
+  return down(y, dy, n);
 }
 
 int kifmm_test_dft_fwd(int np, int nblocks, const double* den, const int* depth, double* spectra) {

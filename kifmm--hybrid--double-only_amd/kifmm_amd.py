@@ -24,6 +24,7 @@ KIFMM_TRGNODE = 2
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EUNSUP"}
 PREC_FP64, PREC_MIXED = 0, 1
 GPU_GRAPH = 1
+GPU_MIXED_INLINE = 2
 PHASE_UP, PHASE_U, PHASE_V, PHASE_W, PHASE_X, PHASE_DOWN, PHASE_ALL = 1, 2, 4, 8, 16, 32, 63
 
 
@@ -80,7 +81,7 @@ class PhaseTimes(C.Structure):
 class WorkCounts(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "u_pairs", "s2m_pairs", "l2t_pairs", "w_pairs", "x_pairs", "v_entries", "v_cmac", "v_cmac_ref",
-        "gemm_flops", "fft_bytes", "hadamard_bytes")] + [("reserved", C.c_double * 5)]
+        "gemm_flops", "fft_bytes", "hadamard_bytes", "close_pairs")] + [("reserved", C.c_double * 4)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -319,11 +320,13 @@ class GpuEvaluator:
     """Device copy of one problem (kifmm_gpu_create) and its evaluation."""
 
     def __init__(self, problem: Problem, device=0, precision=PREC_FP64, operators: Operators | None = None,
-                 graph=False):
+                 graph=False, inline_close=False):
+        """inline_close: on the mixed path, test for close U-list pairs inside
+        the kernel loop instead of listing them at create time."""
         self.problem = problem
         self._view = problem.view()
         self._ops = operators if operators is not None else problem.operators()
-        opts = GpuOptions(device, precision, GPU_GRAPH if graph else 0)
+        opts = GpuOptions(device, precision, (GPU_GRAPH if graph else 0) | (GPU_MIXED_INLINE if inline_close else 0))
         h = C.c_void_p()
         _check(lib.kifmm_gpu_create(C.byref(self._view), C.byref(self._ops), C.byref(opts), C.byref(h)))
         self._h = h

@@ -13,6 +13,7 @@ for name, args in (
     ("kifmm_test_mfma_layout_f32", [C.POINTER(C.c_float)] * 3),
     ("kifmm_test_gemm", [C.c_int, C.c_int, _dp, C.c_int, _dp, _ip, _dp, C.c_int]),
     ("kifmm_test_rsqrt", [C.c_int, _dp, _dp, _dp]),
+    ("kifmm_test_rsq_clamp_f32", [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
     ("kifmm_test_dft_fwd", [C.c_int, C.c_int, _dp, _ip, _dp]),
     ("kifmm_test_dft_inv", [C.c_int, C.c_int, _dp, _dp]),
     ("kifmm_test_translation_spectra", [C.c_int, _dp]),
@@ -64,6 +65,14 @@ def rsqrt(x):
     ym = np.zeros_like(x)
     K._check(lib.kifmm_test_rsqrt(len(x), dp(x), dp(y), dp(ym)))
     return y, ym
+
+
+def rsq_clamp_f32(x):
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros_like(x)
+    fp = lambda a: a.ctypes.data_as(K.C.POINTER(K.C.c_float))
+    K._check(lib.kifmm_test_rsq_clamp_f32(len(x), fp(x), fp(y)))
+    return y
 
 
 def grid_f(np_):

@@ -135,3 +135,19 @@ def test_translation_spectra_match_oracle(np_):
     assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
     near = [(a + 3) + 7 * (b + 3) + 49 * (c + 3) for a in (-1, 0, 1) for b in (-1, 0, 1) for c in (-1, 0, 1)]
     assert np.all(got[:, near] == 0)
+
+
+def test_rsq_clamp_f32():
+    """min(1, 1/sqrt|x|) from v_rsq_f32 with the |x| and clamp modifiers, as
+    the mixed U-list forms it: exactly 1 for |x| <= 1 (0, -0, tiny negative
+    Gram rounding, inf), the raw estimate's accuracy (< 2^-22) beyond."""
+    rng = np.random.default_rng(9)
+    small = np.array([0.0, -0.0, 1e-30, -1e-30, -3e-5, 0.25, 0.999, 1.0, -1.0], dtype=np.float32)
+    y = KL.rsq_clamp_f32(small)
+    assert np.all(y == 1.0), y
+    x = np.concatenate([np.exp(rng.uniform(0.0, np.log(1e30), 100000)), rng.uniform(1.0, 1e4, 100000),
+                        -rng.uniform(1.0, 1e4, 1000)]).astype(np.float32)
+    y = KL.rsq_clamp_f32(x).astype(np.float64)
+    exact = np.minimum(1.0, 1.0 / np.sqrt(np.abs(x.astype(np.float64))))
+    assert np.max(np.abs(y - exact) / exact) < 2.0 ** -22
+    assert KL.rsq_clamp_f32(np.array([np.inf], dtype=np.float32))[0] == 0.0

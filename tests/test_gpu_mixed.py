@@ -1,5 +1,6 @@
 """GPU mixed-precision path (NP <= 4): FP32 Gram-form distances on MFMA,
-FP64 recompute of close pairs, FP32 rsqrt with FP64 accumulation.
+FP32 rsqrt with FP64 accumulation, FP64 terms for the pairs closer than h/8
+(listed once at create time, or tested inside the loop: both variants).
 
 Bar: relative L2 against the FP64 oracle (exact kernel) below 1e-5 on
 identical inputs, and the sampled error against the FP64 direct sum
@@ -30,12 +31,14 @@ def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+@pytest.mark.parametrize("inline", [False, True], ids=["listed", "inline"])
 @pytest.mark.parametrize("case", CASES, ids=lambda c: f"{c[1]}-n{c[0]}-q{c[2]}-np{c[3]}-C{c[5]}")
-def test_mixed_matches_oracle(case):
+def test_mixed_matches_oracle(case, inline):
     n, dist, q, np_, seed, c = case
-    print(f"case n={n} dist={dist} q={q} NP={np_} seed={seed} C={c}", flush=True)
+    print(f"case n={n} dist={dist} q={q} NP={np_} seed={seed} C={c} inline={inline}", flush=True)
     prob = K.Problem.create(n, dist, q, np_=np_, seed=seed, coincide=c)
-    mixed = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+    mixed = K.GpuEvaluator(prob, precision=K.PREC_MIXED, inline_close=inline)
+    assert (mixed.work_counts()["close_pairs"] < 0) == inline
     pm = prob.scatter(mixed.evaluate()[0])
     fp64 = K.GpuEvaluator(prob)
     pd = prob.scatter(fp64.evaluate()[0])
@@ -84,3 +87,49 @@ def test_graph_replays_match_eager():
             assert t["total"] > 0 and t["ulist"] == 0
             again, _ = eager.evaluate()
             assert np.array_equal(again, ref)
+
+
+@pytest.mark.parametrize("n,dist,q,c", [(4000, "uniform", 32, 0), (4000, "ellipsoid", 16, 1), (3000, "gaussmix", 8, 0)])
+def test_close_pair_list_matches_brute_force(n, dist, q, c):
+    """The create-time close-pair list has exactly the U-list pairs with
+    r < h/8, h the target leaf's half-width (r = 0 included), counted here
+    by brute force over the tree's own lists (a 1e-12 margin at the
+    threshold for rounding)."""
+    prob = K.Problem.create(n, dist, q, np_=4, seed=7, coincide=c)
+    ev = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+    got = ev.work_counts()["close_pairs"]
+    t = prob.tree()
+    S = np.stack([t["sx"], t["sy"], t["sz"]], 1)
+    T = np.stack([t["tx"], t["ty"], t["tz"]], 1)
+    lo = hi = 0
+    for i in range(t["n_nodes"]):
+        if t["child"][i] != -1 or t["trg_num"][i] == 0:
+            continue
+        h = 2.0 ** -int(t["depth"][i])
+        tb, tn = t["trg_beg"][i], t["trg_num"][i]
+        for s in t["u_idx"][t["u_ptr"][i]:t["u_ptr"][i + 1]]:
+            sb, sn = t["src_beg"][s], t["src_num"][s]
+            d = np.linalg.norm(T[tb:tb + tn, None, :] - S[None, sb:sb + sn, :], axis=2) / (h / 8)
+            lo += int(np.sum(d < 1.0 - 1e-12))
+            hi += int(np.sum(d < 1.0 + 1e-12))
+    print(f"  close pairs {got:.0f} (brute force {lo}..{hi})")
+    assert lo <= got <= hi and got > 0
+
+
+def test_dense_close_pairs_fall_back_to_the_inline_test():
+    """Groups of 32 points within 1e-6 of each other make every target's
+    close list >= 31 long, over the 16-per-target cap: the evaluator keeps
+    the in-loop test by itself (same result as asking for it) and stays
+    within 1e-5 of the FP64 path."""
+    rng = np.random.default_rng(12)
+    cen = rng.uniform(-0.9, 0.9, (2500, 3))
+    pts = (np.repeat(cen, 32, axis=0) + rng.uniform(-1e-6, 1e-6, (80000, 3))).T
+    den = rng.uniform(-1.0, 1.0, 80000)
+    prob = K.Problem.from_points(*pts, den, pts_max=64, np_=4)
+    auto = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+    assert auto.work_counts()["close_pairs"] == -1
+    a, _ = auto.evaluate()
+    b, _ = K.GpuEvaluator(prob, precision=K.PREC_MIXED, inline_close=True).evaluate()
+    d, _ = K.GpuEvaluator(prob).evaluate()
+    assert np.array_equal(a, b)
+    assert rel(a, d) < 1e-5
