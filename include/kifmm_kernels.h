@@ -23,6 +23,9 @@ int kifmm_test_mfma_layout(const double* a, const double* b, double* out);
 
 /* v_mfma_f32_16x16x4f32 on one wave, same convention (FP32). */
 int kifmm_test_mfma_layout_f32(const float* a, const float* b, float* out);
+/* v_mfma_f32_16x16x32_bf16 probe: lane l passes a[8l..8l+7], b[8l..8l+7]
+ * (converted to bf16); out[4l + r] = accumulator element r of lane l */
+int kifmm_test_mfma_layout_bf16(const float* a, const float* b, float* out);
 
 /* C[:, j] = (acc ? C[:, j] : 0) + 2^sexp[j] * A * B[:, j] through the
  * evaluator's FP64 MFMA GEMM; A [M x K], B [K x ncols], C [M x ncols],

@@ -378,24 +378,84 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
 /* Mixed-precision U-list with the close pairs listed ahead (NP <= 4).
  * Coordinates are centred on the target leaf and scaled by 1 / (tau h)
  * with tau = MIX_CLOSE = 1/8 (a power of two: exact), so a pair is close,
- * r < tau h, exactly when its scaled distance r' is below 1.  The Gram
- * MFMA gives r'^2 as in k_p2p_ulist_mixed; every pair then adds
- * w * min(1, 1 / r') in FP32, the min and a sign guard being the free
- * clamp and |x| modifiers of v_rsq_f32: no per-pair test.  The pairs with
- * r' < 1 in FP64 were found once per plan (k_close_pairs: per target, a
- * CSR list of source indices); when a target's sum is written, each adds
- * w (1 / r' - 1) in FP64 (w (0 - 1) for r = 0 with MASK), which replaces
- * the clamped w by the FP64 term.  Pairs near r' = 1 whose FP32 and FP64
- * sides disagree are off by the FP32 error of a pair at distance tau h
- * (~1e-4 relative to that pair, as for any FP32 pair). */
+ * r < tau h, exactly when its scaled distance r' is below 1.
+ *
+ * Distances: one v_mfma_f32_16x16x32_bf16 per 16 x 16 tile on the Gram form
+ *     r'^2 = |t|^2 + sum_k A[i][k] B[k][j]
+ * with every FP32 operand split exactly into three bf16 parts (v = h + m + l,
+ * split3), k = 9c + 3a + b carrying (-2 t_c)_a (s_c)_b for the 3 axes c,
+ * k = 27..29 carrying 1 * (|s|^2)_b and k = 30, 31 zero: the FP32 Gram
+ * form's accuracy at twice the FP32 MFMA rate (A[i][k] in lane
+ * i + 16(k/8) element k%8, B[k][j] in lane j + 16(k/8), D as for the FP32
+ * MFMA; pinned by test_mfma_bf16_operand_layout).  LDS holds B as
+ * [k/8][source] 16-byte rows, one conflict-free read per lane and chunk.
+ *
+ * Every pair then adds w * min(1, 1 / r') in FP32, the min and a sign guard
+ * being the free clamp and |x| modifiers of v_rsq_f32: no per-pair test.
+ * The pairs with r' < 1 in FP64 were found once per plan (k_close_pairs:
+ * per target, a CSR list of source indices); when a target's sum is
+ * written, each adds w (1 / r' - 1) in FP64 (w (0 - 1) for r = 0 with
+ * MASK), which replaces the clamped w by the FP64 term.  Pairs near r' = 1
+ * whose FP32 and FP64 sides disagree are off by the FP32 error of a pair
+ * at distance tau h (~1e-4 relative to that pair, as for any FP32 pair). */
 constexpr double MIX_CLOSE = 0.125;
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+/* FP32(v) = h + m + l exactly: each remainder is exact in FP32 and the
+ * last has at most 8 significant bits */
+__device__ __forceinline__ void split3(double v, __bf16& h, __bf16& m, __bf16& l) {
+  const float f = (float)v;
+  h = (__bf16)f;
+  const float r1 = f - (float)h;
+  m = (__bf16)r1;
+  l = (__bf16)(r1 - (float)m);
+}
+
+/* B rows of one source at scaled coordinates (x, y, z), |s|^2 = ss */
+__device__ __forceinline__ void gram_b_rows(double x, double y, double z, double ss, v8bf (*sB)[P2P_TILE], int s) {
+  __bf16 p[4][3];
+  split3(x, p[0][0], p[0][1], p[0][2]);
+  split3(y, p[1][0], p[1][1], p[1][2]);
+  split3(z, p[2][0], p[2][1], p[2][2]);
+  split3(ss, p[3][0], p[3][1], p[3][2]);
+#pragma unroll
+  for (int kb = 0; kb < 4; kb++) {
+    v8bf v;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k = 8 * kb + e;
+      v[e] = k < 27 ? p[k / 9][k % 3] : k < 30 ? p[3][k - 27] : (__bf16)0.0f;
+    }
+    sB[kb][s] = v;
+  }
+}
+
+/* A row block kb (k = 8kb .. 8kb + 7) of a target at scaled (x, y, z) */
+__device__ __forceinline__ v8bf gram_a_block(double x, double y, double z, int kb) {
+  __bf16 p[3][3];
+  split3(-2.0 * x, p[0][0], p[0][1], p[0][2]);
+  split3(-2.0 * y, p[1][0], p[1][1], p[1][2]);
+  split3(-2.0 * z, p[2][0], p[2][1], p[2][2]);
+  v8bf out;
+#pragma unroll
+  for (int kc = 0; kc < 4; kc++) {
+    v8bf v;
+#pragma unroll
+    for (int e = 0; e < 8; e++) {
+      const int k = 8 * kc + e;
+      v[e] = k < 27 ? p[k / 9][(k % 9) / 3] : k < 30 ? (__bf16)1.0f : (__bf16)0.0f;
+    }
+    if (kb == kc) out = v;
+  }
+  return out;
+}
 
 template <int G>
-__device__ __forceinline__ void mixed_tau_chunks(const float4* __restrict__ sB, const float* __restrict__ sW,
-                                                 int nchunks, const float* a, const v4f* c, double (*acc)[4],
+__device__ __forceinline__ void mixed_tau_chunks(const v8bf (*sB)[P2P_TILE], const float* __restrict__ sW,
+                                                 int nchunks, const v8bf* a, const v4f* c, double (*acc)[4],
                                                  int lane) {
-  const int j = lane & 15, k4 = lane >> 4;
-  const float* sBf = (const float*)sB;
+  const int j = lane & 15;
+  const v8bf* __restrict__ col = sB[lane >> 4];
   float part[G][4];
 #pragma unroll
   for (int q = 0; q < G; q++)
@@ -405,11 +465,11 @@ __device__ __forceinline__ void mixed_tau_chunks(const float4* __restrict__ sB, 
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int s = (ch + u) * 16 + j;
-      const float b = sBf[4 * s + k4];
+      const v8bf b = col[s];
       const float w = sW[s];
 #pragma unroll
       for (int q = 0; q < G; q++) {
-        const v4f d = __builtin_amdgcn_mfma_f32_16x16x4f32(a[q], b, c[q], 0, 0, 0);
+        const v4f d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, c[q], 0, 0, 0);
 #pragma unroll
         for (int e = 0; e < 4; e++)
           part[q][e] = __builtin_fmaf(
@@ -433,7 +493,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
                                                              const int* __restrict__ cptr,
                                                              const int* __restrict__ cidx,
                                                              double* __restrict__ pot) {
-  __shared__ float4 sB[P2P_TILE];
+  __shared__ v8bf sB[4][P2P_TILE];
   __shared__ float sW[P2P_TILE];
   __shared__ double4 sT[P2P_MAXT];
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -451,7 +511,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
   }
   __syncthreads();
-  float a[2];
+  v8bf a[2];
   v4f c[2];
   double acc[2][4];
 #pragma unroll
@@ -459,7 +519,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     const int grp = min(wave + 4 * q, ngroups - 1);
     const double4 t = sT[16 * grp + (lane & 15)];
     const int k4 = lane >> 4;
-    a[q] = (float)(k4 == 0 ? -2.0 * t.x : k4 == 1 ? -2.0 * t.y : k4 == 2 ? -2.0 * t.z : 1.0);
+    a[q] = gram_a_block(t.x, t.y, t.z, k4);
 #pragma unroll
     for (int e = 0; e < 4; e++) {
       c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
@@ -477,7 +537,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
       for (int k = tid; k < take; k += 256) {
         const double4 q = src4[r.x + ro + k];
         const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
-        sB[fill + k] = make_float4((float)x, (float)y, (float)z, (float)__builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+        gram_b_rows(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)), sB, fill + k);
         sW[fill + k] = (float)q.w;
       }
       fill += take;
@@ -489,7 +549,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     }
     const int ns = (fill + 31) & ~31; /* whole chunk pairs (P2P_TILE is a multiple of 32) */
     for (int k = fill + tid; k < ns; k += 256) {
-      sB[k] = make_float4((float)MIX_FAR, (float)MIX_FAR, (float)MIX_FAR, (float)(3.0 * MIX_FAR * MIX_FAR));
+      gram_b_rows(MIX_FAR, MIX_FAR, MIX_FAR, 3.0 * MIX_FAR * MIX_FAR, sB, k);
       sW[k] = 0.0f;
     }
     __syncthreads();

@@ -205,7 +205,38 @@ __global__ void k_mfma32_probe(const float* a, const float* b, float* out) {
   c = __builtin_amdgcn_mfma_f32_16x16x4f32(a[l], b[l], c, 0, 0, 0);
   for (int r = 0; r < 4; r++) out[4 * l + r] = c[r];
 }
+/* one wave: lane l converts a[8l .. 8l+7], b[8l .. 8l+7] to bf16, passes them
+ * to v_mfma_f32_16x16x32_bf16 and stores the 4 accumulator elements */
+__global__ void k_mfma_bf16_probe(const float* a, const float* b, float* out) {
+  typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+  typedef float v4f __attribute__((ext_vector_type(4)));
+  const int l = threadIdx.x;
+  v8bf va, vb;
+  for (int e = 0; e < 8; e++) {
+    va[e] = (__bf16)a[8 * l + e];
+    vb[e] = (__bf16)b[8 * l + e];
+  }
+  v4f c = {0.0f, 0.0f, 0.0f, 0.0f};
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(va, vb, c, 0, 0, 0);
+  for (int r = 0; r < 4; r++) out[4 * l + r] = c[r];
+}
 }  // namespace
+
+int kifmm_test_mfma_layout_bf16(const float* a, const float* b, float* out) {
+  if (!a || !b || !out) return KIFMM_EINVAL;
+  DevBufs B;
+  float* da = B.up(a, 512);
+  float* db = B.up(b, 512);
+  float* dout = B.up<float>(nullptr, 256);
+  if (!da || !db || !dout) {
+    set_error("device allocation failed");
+    return KIFMM_ENOMEM;
+  }
+  hipLaunchKernelGGL(k_mfma_bf16_probe, dim3(1), dim3(64), 0, 0, da, db, dout);
+  const int r = finish("k_mfma_bf16_probe");
+  if (r != KIFMM_OK) return r;
+  return down(out, dout, 256);
+}
 
 int kifmm_test_mfma_layout_f32(const float* a, const float* b, float* out) {
   if (!a || !b || !out) return KIFMM_EINVAL;

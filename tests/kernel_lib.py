@@ -11,6 +11,7 @@ _ip = C.POINTER(C.c_int)
 for name, args in (
     ("kifmm_test_mfma_layout", [_dp, _dp, _dp]),
     ("kifmm_test_mfma_layout_f32", [C.POINTER(C.c_float)] * 3),
+    ("kifmm_test_mfma_layout_bf16", [C.POINTER(C.c_float)] * 3),
     ("kifmm_test_gemm", [C.c_int, C.c_int, _dp, C.c_int, _dp, _ip, _dp, C.c_int]),
     ("kifmm_test_rsqrt", [C.c_int, _dp, _dp, _dp]),
     ("kifmm_test_rsq_clamp_f32", [C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
@@ -45,6 +46,16 @@ def mfma_layout_f32(a, b):
     out = np.zeros(256, dtype=np.float32)
     fp = C.POINTER(C.c_float)
     K._check(lib.kifmm_test_mfma_layout_f32(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp)))
+    return out.reshape(64, 4)
+
+
+def mfma_layout_bf16(a, b):
+    """a, b: 64 x 8 per-lane values; returns the 64 x 4 accumulator elements"""
+    a = np.ascontiguousarray(a, dtype=np.float32).reshape(512)
+    b = np.ascontiguousarray(b, dtype=np.float32).reshape(512)
+    out = np.zeros(256, dtype=np.float32)
+    fp = C.POINTER(C.c_float)
+    K._check(lib.kifmm_test_mfma_layout_bf16(a.ctypes.data_as(fp), b.ctypes.data_as(fp), out.ctypes.data_as(fp)))
     return out.reshape(64, 4)
 
 

@@ -40,6 +40,28 @@ def test_mfma_f32_operand_layout():
     assert np.array_equal(out, cols + 1.0), f"column map differs:\n{out}"
 
 
+def test_mfma_bf16_operand_layout():
+    """v_mfma_f32_16x16x32_bf16: A[i][k] in lane i + 16(k//8), element k%8;
+    B[k][j] in lane j + 16(k//8), element k%8; D[i][j] in lane j + 16(i//4),
+    element i%4.  Small integers are exact in bf16 and in the FP32 sums, so
+    a random product pins the whole map."""
+    rng = np.random.default_rng(17)
+    A = rng.integers(-4, 5, (16, 32)).astype(np.float32)
+    B = rng.integers(-4, 5, (32, 16)).astype(np.float32)
+    lanes = np.arange(64)
+    a = np.zeros((64, 8), np.float32)
+    b = np.zeros((64, 8), np.float32)
+    for l in lanes:
+        for e in range(8):
+            k = 8 * (l // 16) + e
+            a[l, e] = A[l % 16, k]
+            b[l, e] = B[k, l % 16]
+    out = KL.mfma_layout_bf16(a, b)
+    D = A @ B
+    want = np.array([[D[4 * (l // 16) + r, l % 16] for r in range(4)] for l in lanes])
+    assert np.array_equal(out, want), f"layout differs:\n{out}\n{want}"
+
+
 @pytest.mark.parametrize("M,Kd,n", [(56, 152, 37), (152, 448, 20), (448, 56, 33), (26, 104, 5), (296, 1216, 17),
                                     (152, 160, 64), (152, 448, 9000), (56, 1216, 8200), (98, 56, 300),
                                     (1216, 152, 700), (208, 26, 40)])
