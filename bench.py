@@ -10,7 +10,10 @@ copy of the potentials is reported separately and is not part of `value`.
 
 `mixed_vs_fp64` adds the mixed-precision path against the FP64 path on
 4,194,304 uniform points at NP = 4, 256 points per box (same tree), each
-with its sampled error against the FP64 direct sum.
+with its sampled error against the FP64 direct sum.  `tuned_nonuniform`
+times 1,048,576 pole-clustered ellipsoid points (NP = 4, 128 points per
+box, FP64, one HIP graph) with the default launch shapes and with the
+shapes kifmm_gpu_autotune picks for that input.
 
 Prints one JSON line.  Per-kernel HIP event times come from the library's
 own stream.  --gpus above 1 exits: this build runs on one GPU.
@@ -49,6 +52,7 @@ def parse():
     ap.add_argument("--no-mixed-compare", action="store_true",
                     help="skip the mixed vs FP64 comparison on 4M uniform points at NP = 4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned non-uniform case")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
@@ -147,6 +151,40 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     return out
 
 
+def tuned_nonuniform(K, steps, warmup, nv):
+    """1,048,576 ellipsoid points (semi-axes 0.5, 0.5, 0.125, angles drawn
+    uniformly: clustered at the poles), seed 2, NP = 4, 128 points per box,
+    FP64 in one HIP graph: default launch shapes against the autotuned ones."""
+    n = 1048576
+    prob = K.Problem.create(n, "ellipsoid", 128, np_=4, seed=2, coincide=0)
+    prob.operators()
+    out = {"workload": f"ellipsoid {n} points, separate targets, seed 2, NP=4, 128 pts/box, fp64, HIP graph"}
+    _, ms0, _ = timed(K, prob, K.PREC_FP64, steps, warmup, graph=True)
+    tuner = K.GpuEvaluator(prob)
+    t0 = time.perf_counter()
+    best, cand = tuner.autotune(reps=3)
+    t_tune = time.perf_counter() - t0
+    del tuner
+    ev = K.GpuEvaluator(prob, graph=True)
+    ev.set_tuning(best["p2p_form"], best["hadamard_fb"])
+    for _ in range(warmup):
+        ev.evaluate(copy=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.evaluate(copy=False)
+    ms1 = (time.perf_counter() - t0) / steps * 1e3
+    err = None
+    if nv > 0:
+        err, _, _ = prob.validate_gpu(ev, prob.scatter(ev.evaluate(copy=True)[0]), nv)
+    names = ("p2p_static", "p2p_persistent", "p2p_static64", "hadamard_fb4", "hadamard_fb8", "hadamard_fb16")
+    out.update({"default": {"mpoints_s": n / (ms0 * 1e-3) / 1e6, "ms_per_step": ms0},
+                "tuned": {"mpoints_s": n / (ms1 * 1e-3) / 1e6, "ms_per_step": ms1, "choice": best,
+                          "sampled_rel_err": err},
+                "candidates_ms": {k: round(float(v), 4) for k, v in zip(names, cand)},
+                "autotune_s": t_tune})
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -165,6 +203,9 @@ def main():
     prob.operators()
     t_setup = time.perf_counter() - t0
     setup = prob.setup_times()
+    gprob = K.Problem.create(args.n, args.dist, args.q, np_=args.np_, seed=args.seed, coincide=0, device=0)
+    setup["gpu_tree_lists"] = gprob.setup_times()["tree_lists"]
+    del gprob
     prec = K.PREC_MIXED if args.precision == "mixed" else K.PREC_FP64
     t0 = time.perf_counter()
     ev = K.GpuEvaluator(prob, precision=prec)
@@ -194,6 +235,10 @@ def main():
         del ev
         mixed = mixed_vs_fp64(K, max(5, args.steps // 2), args.warmup, args.nv)
 
+    tuned = None
+    if not args.no_tuned:
+        tuned = tuned_nonuniform(K, max(5, args.steps // 2), args.warmup, args.nv)
+
     cpu = None
     if not args.no_cpu_baseline:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
@@ -220,9 +265,11 @@ def main():
         "d2h_ms": tc["copy"],
         "sampled_rel_err": err,
         "setup_s": {"host_tree_lists": setup["tree_lists"], "generate": setup["generate"],
-                    "operators": setup["operators"], "upload": t_create, "total_host": t_setup},
+                    "operators": setup["operators"], "upload": t_create, "total_host": t_setup,
+                    "gpu_tree_lists": setup["gpu_tree_lists"]},
         "work": {k: wc[k] for k in ("u_pairs", "s2m_pairs", "l2t_pairs", "v_entries", "v_cmac", "gemm_flops")},
         "mixed_vs_fp64": mixed,
+        "tuned_nonuniform": tuned,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

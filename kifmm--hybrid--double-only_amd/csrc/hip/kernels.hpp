@@ -127,39 +127,31 @@ __device__ __forceinline__ int stage_sources(double4* tile, int cap, const doubl
 
 /* ------------------------------------------------------------------ U-list */
 
-/* ------------------------------------------------------------------ U-list */
-
 /* U-list work item = {source range set, first target, target count, leaf},
- * up to P2P_MAXT targets of one target leaf.  A block of 4 waves takes one
+ * up to 4 * 8 * G targets of one target leaf.  A block of 4 waves takes one
  * item: target group g (8 targets) belongs to wave g % 4, so a wave carries
- * up to P2P_G groups of accumulators and every staged source tile serves
- * all the item's targets.  The sources are the leaf's U-list, a list of
- * merged ranges {beg, num} of the gathered source array, walked in order.
- * (The 8 targets of a group are wave-uniform and live in SGPRs: the inner
- * loop is 3 sub + 3 for r^2, v_rsq_f64 + 5 refinement, 1 FMA per pair.) */
+ * up to G groups of accumulators and every staged source tile serves all
+ * the item's targets.  The sources are the leaf's U-list, a list of merged
+ * ranges {beg, num} of the gathered source array, walked in order.  (The 8
+ * targets of a group are wave-uniform and live in SGPRs: the inner loop is
+ * 3 sub + 3 for r^2, v_rsq_f64 + 3 for the Newton step, 1 FMA per pair.)
+ * Per target, the sum is the same for every item size and launch form:
+ * the tiles depend only on the ranges, the lane order and the reduction
+ * are fixed. */
 constexpr int P2P_TILE = 512;
 constexpr int P2P_G = 4;
 constexpr int P2P_MAXT = 4 * 8 * P2P_G;
 
-template <bool MASK>
-__global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items, int n_items, int per_xcd,
-                                                   const int* __restrict__ rng_ptr,
-                                                   const int2* __restrict__ rng,
-                                                   const double4* __restrict__ src4,
-                                                   const double4* __restrict__ trg4,
-                                                   double* __restrict__ pot) {
-  __shared__ double4 tile[P2P_TILE];
-  /* workgroups go round-robin to the 8 XCDs: give each XCD a contiguous run
-   * of items (neighbouring leaves) so their shared sources stay in its L2 */
-  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (item >= n_items) return; /* whole block, before any barrier */
-  const int4 it = items[item];
+template <bool MASK, int G>
+__device__ __forceinline__ void p2p_item(double4* tile, const int4 it, const int* __restrict__ rng_ptr,
+                                         const int2* __restrict__ rng, const double4* __restrict__ src4,
+                                         const double4* __restrict__ trg4, double* __restrict__ pot) {
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ngroups = (it.z + 7) >> 3;
-  double acc[P2P_G][8];
+  double acc[G][8];
 #pragma unroll
-  for (int k = 0; k < P2P_G; k++)
+  for (int k = 0; k < G; k++)
 #pragma unroll
     for (int t = 0; t < 8; t++) acc[k][t] = 0.0;
   int ri = rng_ptr[it.x], ro = 0;
@@ -184,7 +176,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
     for (int k = fill + tid; k < ns; k += 256) tile[k] = make_double4(FAR, FAR, FAR, 0.0);
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < P2P_G; k++) {
+    for (int k = 0; k < G; k++) {
       const int grp = wave + 4 * k;
       if (grp < ngroups) {
         /* slots past the count re-read the item's last target (in range) */
@@ -199,7 +191,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
     }
   }
 #pragma unroll
-  for (int k = 0; k < P2P_G; k++) {
+  for (int k = 0; k < G; k++) {
     const int grp = wave + 4 * k;
     if (grp < ngroups) {
       const double sum = wave_reduce8(acc[k], lane);
@@ -207,6 +199,51 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
       if ((lane & 7) == 0 && t < it.z) pot[it.y + t] = HALF_OOFP_R * sum;
     }
   }
+}
+
+/* One block per item of at most 32 G targets. */
+template <bool MASK, int G>
+__global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                   const int* __restrict__ rng_ptr, const int2* __restrict__ rng,
+                                                   const double4* __restrict__ src4,
+                                                   const double4* __restrict__ trg4, double* __restrict__ pot) {
+  __shared__ double4 tile[P2P_TILE];
+  /* workgroups go round-robin to the 8 XCDs: give each XCD a contiguous run
+   * of items (neighbouring leaves) so their shared sources stay in its L2 */
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  p2p_item<MASK, G>(tile, items[item], rng_ptr, rng, src4, trg4, pot);
+}
+
+/* Persistent form over the same items (<= P2P_MAXT targets): a fixed grid
+ * of a few blocks per CU; the blocks of XCD x draw the items of x's
+ * contiguous run one at a time from counter x (zeroed before the launch),
+ * so uneven items even out inside each XCD.  Every block leaves the loop
+ * at its first draw past the run's end; the counters only grow. */
+template <bool MASK>
+__global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                           const int* __restrict__ rng_ptr,
+                                                           const int2* __restrict__ rng,
+                                                           const double4* __restrict__ src4,
+                                                           const double4* __restrict__ trg4, double* __restrict__ pot,
+                                                           int* __restrict__ counters) {
+  __shared__ double4 tile[P2P_TILE];
+  __shared__ int draw;
+  const int x = (int)(blockIdx.x & 7);
+  const int lo = x * per_xcd, n = min(n_items - lo, per_xcd);
+  for (;;) {
+    if (threadIdx.x == 0) draw = atomicAdd(counters + x, 1);
+    __syncthreads();
+    const int k = draw;
+    __syncthreads(); /* every thread has read draw before the next draw overwrites it */
+    if (k >= n) break;
+    p2p_item<MASK, P2P_G>(tile, items[lo + k], rng_ptr, rng, src4, trg4, pot);
+  }
+}
+
+static __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = v;
 }
 
 /* ------------------------------------------------- U-list, mixed precision */
@@ -1172,12 +1209,11 @@ __device__ __forceinline__ void hadamard_half(const ST (*Xs)[9], const unsigned 
 }
 
 /* One block = one parent group (up to 64 parents, a lane each) and
- * HAD_FB consecutive frequencies.  The union of the group's neighbour
+ * FB consecutive frequencies (4, 8 or 16, chosen by the autotuner).  The union of the group's neighbour
  * slots is staged in LDS per frequency; the next frequency's spectra are
  * loaded into registers while the current one is multiplied, and the
  * parents' 27 union indices are staged once per block. */
-constexpr int HAD_FB = 8;
-template <int NP, typename ST = double2>
+template <int NP, typename ST, int FB>
 __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ groups, /* {first parent, count, union beg, union size} */
                                                   const int* __restrict__ uslots,  /* X slots of each union */
                                                   const unsigned char* __restrict__ loc, /* 27 union indices per parent */
@@ -1188,12 +1224,12 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
   typedef typename Spec<ST>::R R;
   using G = Grid<NP>;
   constexpr int F = G::F;
-  static_assert(F % HAD_FB == 0, "frequency blocks");
+  static_assert(F % FB == 0, "frequency blocks");
   constexpr int PER = (HAD_CAP * 8 + 127) / 128;
   __shared__ ST Xs[HAD_CAP][9];
   __shared__ unsigned char sloc[27][64];
   const int4 g = groups[blockIdx.x];
-  const int f0 = blockIdx.y * HAD_FB;
+  const int f0 = blockIdx.y * FB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int half = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ne8 = g.w * 8;
@@ -1220,7 +1256,7 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
   }
   const int p = g.x + min(lane, g.y - 1);
   const int ys = yslot[p];
-  for (int fi = 0; fi < HAD_FB; fi++) {
+  for (int fi = 0; fi < FB; fi++) {
     const int f = f0 + fi;
     __syncthreads(); /* the previous frequency's readers are done with Xs */
 #pragma unroll
@@ -1229,7 +1265,7 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
       if (e < ne8) Xs[e >> 3][e & 7] = pre[k];
     }
     __syncthreads();
-    if (fi + 1 < HAD_FB) fetch(f + 1);
+    if (fi + 1 < FB) fetch(f + 1);
     const ST* __restrict__ Tf = T + (size_t)f * 343;
     R yr[4] = {0, 0, 0, 0}, yi[4] = {0, 0, 0, 0};
     if (half == 0)

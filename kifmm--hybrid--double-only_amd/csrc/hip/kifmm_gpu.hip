@@ -78,7 +78,13 @@ struct kifmm_gpu {
   int *wptr = nullptr, *widx = nullptr, *xptr = nullptr, *xidx = nullptr;
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
+  int4* p2p_items64 = nullptr; int n_p2p64 = 0;
   int* rng_ptr = nullptr; int2* rng = nullptr;
+  int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
+  int persist_grid = 0;
+  /* launch shapes (kifmm_gpu_set_tuning / kifmm_gpu_autotune) */
+  int p2p_form = KIFMM_P2P_STATIC;
+  int had_fb = 8;
   /* mixed path: per target, the U-list sources closer than MIX_CLOSE h */
   int* cptr = nullptr; int* cidx = nullptr;
   int close_listed = 0;
@@ -218,7 +224,7 @@ struct HostLevel {
 struct Plan {
   int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
   int N = 0, H = 0, F = 0, nblk = 0, max_depth = 0;
-  std::vector<int4> p2p_items;
+  std::vector<int4> p2p_items, p2p_items64;
   std::vector<int> rng_ptr;
   std::vector<int2> rng;
   std::vector<int4> s2m_items;
@@ -284,16 +290,19 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     }
     P.rng_ptr.push_back((int)P.rng.size());
     const int n = t->trg_num[i];
-    const int nit = (n + P2P_MAXT - 1) / P2P_MAXT;
-    const int per = (((n + nit - 1) / nit) + 15) & ~15;
-    for (int c = 0; c < n; c += per)
-      P.p2p_items.push_back(make_int4(set, t->trg_beg[i] + c, std::min(per, n - c), i));
+    for (int cap : {P2P_MAXT, P2P_MAXT / 2}) { /* the item lists of both launch shapes */
+      const int nit = (n + cap - 1) / cap;
+      const int per = (((n + nit - 1) / nit) + 15) & ~15;
+      std::vector<int4>& out = cap == P2P_MAXT ? P.p2p_items : P.p2p_items64;
+      for (int c = 0; c < n; c += per) out.push_back(make_int4(set, t->trg_beg[i] + c, std::min(per, n - c), i));
+    }
   }
   /* k_p2p_ulist reads trg4[it.y .. it.y+it.z-1] and src4 inside each range */
-  for (const int4& it : P.p2p_items)
-    if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > P2P_MAXT || it.y < 0 ||
-        (long)it.y + it.z > t->n_trg || it.w < 0 || it.w >= nn)
-      return fail("U-list work item out of range");
+  for (int cap : {P2P_MAXT, P2P_MAXT / 2})
+    for (const int4& it : cap == P2P_MAXT ? P.p2p_items : P.p2p_items64)
+      if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > cap || it.y < 0 ||
+          (long)it.y + it.z > t->n_trg || it.w < 0 || it.w >= nn)
+        return fail("U-list work item out of range");
   for (const int2& r : P.rng)
     if (r.x < 0 || r.y < 1 || (long)r.x + r.y > t->n_src) return fail("U-list source range out of range");
 
@@ -602,6 +611,15 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   /* work lists */
   g->n_p2p = (int)P.p2p_items.size();
   TRY(dev_upload(g, &g->p2p_items, P.p2p_items.data(), P.p2p_items.size()));
+  g->n_p2p64 = (int)P.p2p_items64.size();
+  TRY(dev_upload(g, &g->p2p_items64, P.p2p_items64.data(), P.p2p_items64.size()));
+  TRY(dev_zero(g, &g->p2p_counters, 8));
+  {
+    int ncu = 0, per_cu = 0;
+    HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false>, 256, 0));
+    g->persist_grid = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
+  }
   TRY(dev_upload(g, &g->rng_ptr, P.rng_ptr.data(), P.rng_ptr.size()));
   TRY(dev_upload(g, &g->rng, P.rng.data(), P.rng.size()));
   g->n_s2m = (int)P.s2m_items.size();
@@ -731,8 +749,19 @@ int vlist_phase(kifmm_gpu* g, ST* X, ST* Y, ST* T, hipEvent_t e1, hipEvent_t e2,
   LAUNCH_CHECK();
   REC(e2);
   if (g->n_hgroups > 0)
-    hipLaunchKernelGGL((k_hadamard<NP, ST>), dim3(g->n_hgroups, g->F / HAD_FB), dim3(128), 0, g->stream, g->hgroups,
-                       g->huslots, g->hloc, g->hslot, T, X, Y);
+    switch (g->had_fb) {
+      case 4:
+        hipLaunchKernelGGL((k_hadamard<NP, ST, 4>), dim3(g->n_hgroups, g->F / 4), dim3(128), 0, g->stream,
+                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
+        break;
+      case 16:
+        hipLaunchKernelGGL((k_hadamard<NP, ST, 16>), dim3(g->n_hgroups, g->F / 16), dim3(128), 0, g->stream,
+                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
+        break;
+      default:
+        hipLaunchKernelGGL((k_hadamard<NP, ST, 8>), dim3(g->n_hgroups, g->F / 8), dim3(128), 0, g->stream,
+                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
+    }
   LAUNCH_CHECK();
   REC(e3);
   if (g->n_vblocks > 0)
@@ -759,9 +788,17 @@ int point_phases_u(kifmm_gpu* g) {
     else if (g->precision == KIFMM_PREC_MIXED)
       hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
                          g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
-    else
-      hipLaunchKernelGGL(k_p2p_ulist<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items, g->n_p2p,
-                         per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
+    else if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
+      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, g->stream, g->p2p_counters, 8, 0);
+      hipLaunchKernelGGL(k_p2p_ulist_persist<MASK>, dim3(g->persist_grid), dim3(256), 0, g->stream, g->p2p_items,
+                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters);
+    } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
+      const int per64 = (g->n_p2p64 + 7) / 8;
+      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G / 2>), dim3(8 * per64), dim3(256), 0, g->stream, g->p2p_items64,
+                         g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
+    } else
+      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G>), dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
+                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
   }
   LAUNCH_CHECK();
   return KIFMM_OK;
@@ -1118,6 +1155,87 @@ int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifm
   }
   if (r == KIFMM_OK && out) *out = P.wc;
   return r;
+}
+
+int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
+  if (!g || !t) {
+    set_error("kifmm_gpu_set_tuning: null argument");
+    return KIFMM_EINVAL;
+  }
+  if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_STATIC64 ||
+      (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16)) {
+    set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value and hadamard_fb 4, 8 or 16");
+    return KIFMM_EINVAL;
+  }
+  if (t->p2p_form != KIFMM_P2P_STATIC && g->precision != KIFMM_PREC_FP64) {
+    set_error("kifmm_gpu_set_tuning: the U-list forms apply to the FP64 path");
+    return KIFMM_EUNSUP;
+  }
+  if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb) return KIFMM_OK;
+  g->p2p_form = t->p2p_form;
+  g->had_fb = t->hadamard_fb;
+  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  g->gexec = nullptr;
+  g->graph = nullptr;
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t) {
+  if (!g || !t) {
+    set_error("kifmm_gpu_get_tuning: null argument");
+    return KIFMM_EINVAL;
+  }
+  std::memset(t, 0, sizeof *t);
+  t->p2p_form = g->p2p_form;
+  t->hadamard_fb = g->had_fb;
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms) {
+  if (!g || reps < 1 || (ms && n_ms < 0)) {
+    set_error("kifmm_gpu_autotune: null handle or reps < 1");
+    return KIFMM_EINVAL;
+  }
+  double t[6] = {-1, -1, -1, -1, -1, -1};
+  kifmm_gpu_tuning cur{};
+  kifmm_gpu_get_tuning(g, &cur);
+  auto time_phase = [&](unsigned phase, double* out) -> int {
+    kifmm_phase_times pt{};
+    double m = 0.0;
+    for (int r = 0; r <= reps; r++) {
+      const int rc = evaluate(g, phase, nullptr, &pt);
+      if (rc != KIFMM_OK) return rc;
+      const double v = phase == KIFMM_PHASE_U ? pt.ulist : pt.v_hadamard;
+      if (r == 1 || (r > 1 && v < m)) m = v; /* run 0 warms up */
+    }
+    *out = m;
+    return KIFMM_OK;
+  };
+  kifmm_gpu_tuning c = cur;
+  const int nforms = g->precision == KIFMM_PREC_FP64 ? 3 : 1;
+  int best_form = 0;
+  for (int f = 0; f < nforms; f++) {
+    c.p2p_form = f;
+    TRY(kifmm_gpu_set_tuning(g, &c));
+    TRY(time_phase(KIFMM_PHASE_U, &t[f]));
+    if (t[f] < t[best_form]) best_form = f;
+  }
+  c.p2p_form = best_form;
+  const int fbs[3] = {4, 8, 16};
+  int best_fb = 0;
+  for (int k = 0; k < 3; k++) {
+    c.hadamard_fb = fbs[k];
+    TRY(kifmm_gpu_set_tuning(g, &c));
+    TRY(time_phase(KIFMM_PHASE_V, &t[3 + k]));
+    if (t[3 + k] < t[3 + best_fb]) best_fb = k;
+  }
+  c.hadamard_fb = fbs[best_fb];
+  TRY(kifmm_gpu_set_tuning(g, &c));
+  if (best) *best = c;
+  if (ms)
+    for (int k = 0; k < std::min(n_ms, 6); k++) ms[k] = t[k];
+  return KIFMM_OK;
 }
 
 int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out) {

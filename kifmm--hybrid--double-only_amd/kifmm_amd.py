@@ -69,6 +69,13 @@ class GpuOptions(C.Structure):
     _fields_ = [("device", C.c_int), ("precision", C.c_int), ("flags", C.c_int), ("reserved", C.c_int * 5)]
 
 
+class GpuTuning(C.Structure):
+    _fields_ = [("p2p_form", C.c_int), ("hadamard_fb", C.c_int), ("reserved", C.c_int * 6)]
+
+
+P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64 = 0, 1, 2
+
+
 class PhaseTimes(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "up", "ulist", "vlist", "v_fft_src", "v_trans", "v_hadamard", "v_ifft", "wlist", "xlist",
@@ -125,6 +132,9 @@ _sig("kifmm_gpu_evaluate_phases", C.c_int, [C.c_void_p, C.c_uint, _dp, C.POINTER
 _sig("kifmm_gpu_work_counts", C.c_int, [C.c_void_p, C.POINTER(WorkCounts)])
 _sig("kifmm_gpu_copy_intermediate", C.c_int, [C.c_void_p, C.c_int, _dp, C.c_long])
 _sig("kifmm_gpu_plan", C.c_int, [C.POINTER(TreeView), C.POINTER(Operators), C.POINTER(WorkCounts)])
+_sig("kifmm_gpu_set_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
+_sig("kifmm_gpu_get_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
+_sig("kifmm_gpu_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(GpuTuning), _dp, C.c_int])
 
 
 def last_error() -> str:
@@ -357,6 +367,23 @@ class GpuEvaluator:
         pot = np.zeros(self.n_trg) if copy else None
         _check(lib.kifmm_gpu_evaluate_phases(self._h, phases, _dptr(pot) if copy else None, C.byref(t)))
         return pot, t.as_dict()
+
+    def tuning(self):
+        t = GpuTuning()
+        _check(lib.kifmm_gpu_get_tuning(self._h, C.byref(t)))
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb}
+
+    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8):
+        t = GpuTuning(p2p_form, hadamard_fb)
+        _check(lib.kifmm_gpu_set_tuning(self._h, C.byref(t)))
+
+    def autotune(self, reps=3):
+        """Time the launch shapes on this input, keep the fastest; returns
+        (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, fb 4, 8, 16."""
+        t = GpuTuning()
+        ms = np.zeros(6)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 6))
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb}, ms
 
     def work_counts(self):
         w = WorkCounts()
