@@ -10,7 +10,9 @@ copy of the potentials is reported separately and is not part of `value`.
 
 `mixed_vs_fp64` adds the mixed-precision path against the FP64 path on
 4,194,304 uniform points at NP = 4, 256 points per box (same tree), each
-with its sampled error against the FP64 direct sum.  `tuned_nonuniform`
+with its sampled error against the FP64 direct sum, and the model-tuned
+path (cost model's q and precision, autotuned launch shapes, one HIP
+graph); `ratio` is that path over the plain FP64 path.  `tuned_nonuniform`
 times 1,048,576 pole-clustered ellipsoid points (NP = 4, 128 points per
 box, FP64, one HIP graph) with the default launch shapes and with the
 shapes kifmm_gpu_autotune picks for that input.
@@ -145,8 +147,38 @@ def mixed_vs_fp64(K, steps, warmup, nv):
         if not graph:
             out[name]["phases_ms"] = {k: round(v, 4) for k, v in avg.items()}
         del ev
-    # the mixed, batched (one HIP graph) path against the plain FP64 path
-    out["ratio"] = out["mixed_graph"]["mpoints_s"] / out["fp64"]["mpoints_s"]
+    # the model-tuned path: the cost model picks q and the precision path for
+    # this input, kifmm_gpu_autotune the launch shapes, one HIP graph per step
+    import costmodel as CM
+    pts = prob.points()
+    pts = tuple(pts[k] for k in ("sx", "sy", "sz", "den", "tx", "ty", "tz"))
+    t0 = time.perf_counter()
+    q, prec, pred, how = CM.Tuner(CM.CostModel.load()).choose(pts, 4)
+    t_choose = time.perf_counter() - t0
+    mprob = K.Problem.from_points(*pts, pts_max=q, np_=4)
+    mprob.operators()
+    tuner = K.GpuEvaluator(mprob, precision=prec)
+    shapes, _ = tuner.autotune(reps=3)
+    del tuner
+    ev = K.GpuEvaluator(mprob, precision=prec, graph=True)
+    ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"])
+    for _ in range(warmup):
+        ev.evaluate(copy=False)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ev.evaluate(copy=False)
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    err = None
+    if nv > 0 and sample is not None:
+        f = mprob.scatter(ev.evaluate(copy=True)[0])[sample[0]]
+        err = float(np.linalg.norm(sample[1] - f) / np.linalg.norm(f))
+    out["model_tuned"] = {"mpoints_s": n / (ms * 1e-3) / 1e6, "ms_per_step": ms, "sampled_rel_err": err,
+                          "q": q, "precision": "mixed" if prec == K.PREC_MIXED else "fp64", "shapes": shapes,
+                          "chosen_by": how, "predicted_ms": pred, "choose_s": t_choose}
+    del ev
+    # the mixed-precision, batched, model-tuned path against the plain FP64 path
+    out["ratio"] = out["model_tuned"]["mpoints_s"] / out["fp64"]["mpoints_s"]
+    out["ratio_mixed_graph_default_q"] = out["mixed_graph"]["mpoints_s"] / out["fp64"]["mpoints_s"]
     out["ratio_both_graphed"] = out["mixed_graph"]["mpoints_s"] / out["fp64_graph"]["mpoints_s"]
     return out
 
