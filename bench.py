@@ -55,6 +55,7 @@ def parse():
                     help="skip the mixed vs FP64 comparison on 4M uniform points at NP = 4")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned non-uniform case")
+    ap.add_argument("--no-gpu-tree", action="store_true", help="skip timing the GPU tree build")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
@@ -235,9 +236,11 @@ def main():
     prob.operators()
     t_setup = time.perf_counter() - t0
     setup = prob.setup_times()
-    gprob = K.Problem.create(args.n, args.dist, args.q, np_=args.np_, seed=args.seed, coincide=0, device=0)
-    setup["gpu_tree_lists"] = gprob.setup_times()["tree_lists"]
-    del gprob
+    setup["gpu_tree_lists"] = None
+    if not args.no_gpu_tree:
+        gprob = K.Problem.create(args.n, args.dist, args.q, np_=args.np_, seed=args.seed, coincide=0, device=0)
+        setup["gpu_tree_lists"] = gprob.setup_times()["tree_lists"]
+        del gprob
     prec = K.PREC_MIXED if args.precision == "mixed" else K.PREC_FP64
     t0 = time.perf_counter()
     ev = K.GpuEvaluator(prob, precision=prec)
