@@ -143,11 +143,14 @@ constexpr int P2P_G = 4;
 constexpr int P2P_MAXT = 4 * 8 * P2P_G;
 
 template <bool MASK, int G>
-__device__ __forceinline__ void p2p_item(double4* tile, const int4 it, const int* __restrict__ rng_ptr,
+__device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, const int* __restrict__ rng_ptr,
                                          const int2* __restrict__ rng, const double4* __restrict__ src4,
                                          const double4* __restrict__ trg4, double* __restrict__ pot) {
   const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  /* groups g = wave' + 4k with wave' the wave index rotated by the item:
+   * when the group count is not a multiple of 4 the short wave moves from
+   * SIMD to SIMD instead of always idling the same one */
+  const int wave = (__builtin_amdgcn_readfirstlane(tid >> 6) + rot) & 3;
   const int ngroups = (it.z + 7) >> 3;
   double acc[G][8];
 #pragma unroll
@@ -212,7 +215,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
    * of items (neighbouring leaves) so their shared sources stay in its L2 */
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
-  p2p_item<MASK, G>(tile, items[item], rng_ptr, rng, src4, trg4, pot);
+  p2p_item<MASK, G>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot);
 }
 
 /* Persistent form over the same items (<= P2P_MAXT targets): a fixed grid
@@ -237,7 +240,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restric
     const int k = draw;
     __syncthreads(); /* every thread has read draw before the next draw overwrites it */
     if (k >= n) break;
-    p2p_item<MASK, P2P_G>(tile, items[lo + k], rng_ptr, rng, src4, trg4, pot);
+    p2p_item<MASK, P2P_G>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot);
   }
 }
 
@@ -539,7 +542,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
   const double4 g = geo[it.w];
   const double sc = (1.0 / MIX_CLOSE) / g.w; /* 1 / (tau h), a power of two */
   const int tid = threadIdx.x, lane = tid & 63;
-  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int wave = (__builtin_amdgcn_readfirstlane(tid >> 6) + item) & 3; /* rotated as in p2p_item */
   const int ngroups = (it.z + 15) >> 4;
   /* scaled targets; slots past the count repeat the last target */
   for (int k = tid; k < ngroups * 16; k += 256) {
@@ -940,24 +943,26 @@ struct GemmShape {
 inline GemmShape gemm_shape(int ncols, int M, int K) {
   GemmShape s;
   const int tiles = (M + 15) / 16;
-  if (tiles <= 10) {
-    s.ct = 2;
-    s.rt = tiles <= 2 ? 2 : tiles <= 4 ? 4 : tiles <= 7 ? 7 : 10;
-    s.row_groups = 1;
-  } else {
-    s.ct = 1;
-    s.row_groups = (tiles + 19) / 20;
-    const int per = (tiles + s.row_groups - 1) / s.row_groups;
-    s.rt = per <= 14 ? 14 : 20;
-  }
+  /* short operators: two 64-column tiles per block; otherwise one, with the
+   * rows split into groups of <= 10 tiles.  When the inputs are short
+   * (K <= 2M: re-reading them per row group is cheap) the rows are split
+   * further, down to 4 tiles, while the launch has fewer than 1024 blocks
+   * (4 per CU); long inputs are split along K instead (below). */
+  s.ct = tiles <= 4 ? 2 : 1;
+  const int cb = (ncols + 64 * s.ct - 1) / (64 * s.ct);
+  s.row_groups = (tiles + 9) / 10;
+  while (s.ct == 1 && K <= 2 * M && cb * s.row_groups < 1024 && (tiles + s.row_groups) / (s.row_groups + 1) >= 4)
+    s.row_groups++;
+  const int per = (tiles + s.row_groups - 1) / s.row_groups;
+  s.rt = s.ct == 2 ? (per <= 2 ? 2 : 4) : per <= 4 ? 4 : per <= 5 ? 5 : per <= 7 ? 7 : 10;
   s.col_blocks = (ncols + 64 * s.ct - 1) / (64 * s.ct);
   const int blocks = s.col_blocks * s.row_groups;
   const int kchunks = (K + GEMM_KC - 1) / GEMM_KC;
   int ns = 1;
   if (blocks < 256) ns = min(min(16, kchunks), (512 + blocks - 1) / blocks);
   ns = max(ns, 1);
-  const int per = (kchunks + ns - 1) / ns;
-  s.ksplit = per * GEMM_KC;
+  const int kper = (kchunks + ns - 1) / ns;
+  s.ksplit = kper * GEMM_KC;
   s.nsplit = (K + s.ksplit - 1) / s.ksplit;
   s.ldw = (M + 7) & ~7;
   return s;
@@ -977,10 +982,10 @@ inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, i
                        in, out, accumulate, ks, w, sh.ldw);
   KIFMM_GEMM_CASE(2, 2)
   else KIFMM_GEMM_CASE(4, 2)
-  else KIFMM_GEMM_CASE(7, 2)
-  else KIFMM_GEMM_CASE(10, 2)
-  else KIFMM_GEMM_CASE(14, 1)
-  else KIFMM_GEMM_CASE(20, 1)
+  else KIFMM_GEMM_CASE(4, 1)
+  else KIFMM_GEMM_CASE(5, 1)
+  else KIFMM_GEMM_CASE(7, 1)
+  else KIFMM_GEMM_CASE(10, 1)
   else return hipErrorInvalidValue;
 #undef KIFMM_GEMM_CASE
   hipError_t e = hipGetLastError();
