@@ -15,7 +15,8 @@ path (cost model's q and precision, autotuned launch shapes, one HIP
 graph); `ratio` is that path over the plain FP64 path.  `tuned_nonuniform`
 times 1,048,576 pole-clustered ellipsoid points (NP = 4, 128 points per
 box, FP64, one HIP graph) with the default launch shapes and with the
-shapes kifmm_gpu_autotune picks for that input.
+shapes kifmm_gpu_autotune picks for that input.  `large_mixed` runs
+16,777,216 uniform points (256 per box) at NP = 3 and 4 on both paths.
 
 Prints one JSON line.  Per-kernel HIP event times come from the library's
 own stream.  --gpus above 1 exits: this build runs on one GPU.
@@ -56,6 +57,7 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned non-uniform case")
     ap.add_argument("--no-gpu-tree", action="store_true", help="skip timing the GPU tree build")
+    ap.add_argument("--no-large", action="store_true", help="skip the 16M-point mixed-vs-FP64 case")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
@@ -162,7 +164,7 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     shapes, _ = tuner.autotune(reps=3)
     del tuner
     ev = K.GpuEvaluator(mprob, precision=prec, graph=True)
-    ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"])
+    ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"], shapes["hadamard_form"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
     t0 = time.perf_counter()
@@ -199,7 +201,7 @@ def tuned_nonuniform(K, steps, warmup, nv):
     t_tune = time.perf_counter() - t0
     del tuner
     ev = K.GpuEvaluator(prob, graph=True)
-    ev.set_tuning(best["p2p_form"], best["hadamard_fb"])
+    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
     t0 = time.perf_counter()
@@ -209,12 +211,43 @@ def tuned_nonuniform(K, steps, warmup, nv):
     err = None
     if nv > 0:
         err, _, _ = prob.validate_gpu(ev, prob.scatter(ev.evaluate(copy=True)[0]), nv)
-    names = ("p2p_static", "p2p_persistent", "p2p_static64", "hadamard_fb4", "hadamard_fb8", "hadamard_fb16")
+    names = ("p2p_static", "p2p_persistent", "p2p_static64", "hadamard_valu_fb4", "hadamard_valu_fb8",
+             "hadamard_valu_fb16", "hadamard_mfma_fb4", "hadamard_mfma_fb8", "hadamard_mfma_fb16")
     out.update({"default": {"mpoints_s": n / (ms0 * 1e-3) / 1e6, "ms_per_step": ms0},
                 "tuned": {"mpoints_s": n / (ms1 * 1e-3) / 1e6, "ms_per_step": ms1, "choice": best,
                           "sampled_rel_err": err},
                 "candidates_ms": {k: round(float(v), 4) for k, v in zip(names, cand)},
                 "autotune_s": t_tune})
+    return out
+
+
+def large_mixed(K, steps, warmup, nv):
+    """16,777,216 uniform points, seed 3, 256 points per box, at NP = 3 and 4:
+    the plain FP64 path and the mixed path (one HIP graph each), Mpoints/s,
+    sampled error against the FP64 direct sum and the speedup.  The tree and
+    lists are built on the GPU (identical to the host build)."""
+    n = 16777216
+    out = {"workload": f"uniform {n} points, separate targets, seed 3, 256 pts/box"}
+    for np_ in (3, 4):
+        prob = K.Problem.create(n, "uniform", 256, np_=np_, seed=3, coincide=0, device=0)
+        prob.operators()
+        row, sample = {}, None
+        for name, prec in (("fp64", K.PREC_FP64), ("mixed", K.PREC_MIXED)):
+            ev, ms, _ = timed(K, prob, prec, steps, warmup, graph=True)
+            err = None
+            if nv > 0:
+                pot = prob.scatter(ev.evaluate(copy=True)[0])
+                if sample is None:
+                    err, ids, direct = prob.validate_gpu(ev, pot, nv)
+                    sample = (ids, direct)
+                else:
+                    f = pot[sample[0]]
+                    err = float(np.linalg.norm(sample[1] - f) / np.linalg.norm(f))
+            row[name] = {"mpoints_s": n / (ms * 1e-3) / 1e6, "ms_per_step": ms, "sampled_rel_err": err}
+            del ev
+        row["speedup"] = row["mixed"]["mpoints_s"] / row["fp64"]["mpoints_s"]
+        out[f"np{np_}"] = row
+        del prob
     return out
 
 
@@ -270,6 +303,10 @@ def main():
         del ev
         mixed = mixed_vs_fp64(K, max(5, args.steps // 2), args.warmup, args.nv)
 
+    large = None
+    if not args.no_large:
+        large = large_mixed(K, max(3, args.steps // 4), 2, args.nv)
+
     tuned = None
     if not args.no_tuned:
         tuned = tuned_nonuniform(K, max(5, args.steps // 2), args.warmup, args.nv)
@@ -305,6 +342,7 @@ def main():
         "work": {k: wc[k] for k in ("u_pairs", "s2m_pairs", "l2t_pairs", "v_entries", "v_cmac", "gemm_flops")},
         "mixed_vs_fp64": mixed,
         "tuned_nonuniform": tuned,
+        "large_mixed": large,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

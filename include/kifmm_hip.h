@@ -51,10 +51,15 @@ enum {
   KIFMM_P2P_STATIC64 = 2    /* one block per item of <= 64 targets */
 };
 typedef struct kifmm_gpu_tuning {
-  int p2p_form;    /* KIFMM_P2P_*; the FP64 U-list only (the mixed path has one form) */
-  int hadamard_fb; /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
-  int reserved[6];
+  int p2p_form;      /* KIFMM_P2P_*; the FP64 U-list only (the mixed path has one form) */
+  int hadamard_fb;   /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
+  int hadamard_form; /* KIFMM_HAD_*: V-list Hadamard on MFMA (default) or on the VALU */
+  int reserved[5];
 } kifmm_gpu_tuning;
+enum {
+  KIFMM_HAD_VALU = 0, /* a lane per parent, two waves per 64 parents */
+  KIFMM_HAD_MFMA = 1  /* 16 parents per 16x16x4 MFMA tile, four waves per 64 parents (default) */
+};
 
 typedef struct kifmm_gpu_options {
   int device;    /* HIP device ordinal */
@@ -134,18 +139,20 @@ int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifm
  * (n_trg doubles, gathered order). */
 int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long count);
 
-/* Set or read the launch shapes.  Every shape gives bit-identical
- * potentials.  Setting drops a captured graph (the next full evaluate
- * captures again).  EINVAL for an unknown value, EUNSUP for a U-list form
- * other than STATIC on the mixed path. */
+/* Set or read the launch shapes.  The U-list forms and the Hadamard block
+ * sizes give bit-identical potentials; the two Hadamard forms sum in
+ * different orders (rounding-level differences).  Setting drops a captured
+ * graph (the next full evaluate captures again).  EINVAL for an unknown
+ * value, EUNSUP for a U-list form other than STATIC on the mixed path. */
 int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t);
 int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
 /* Time every U-list form (FP64 path) on the U phase and every Hadamard
- * block size on the V phase of this handle's own input, reps timed runs
- * each after one warm-up, keep the fastest of each (minimum over runs) and
- * set it.  best (may be NULL) receives the choice; ms (may be NULL, n_ms
- * entries) receives the times in the order STATIC, PERSISTENT, STATIC64,
- * fb 4, fb 8, fb 16, -1 for a candidate not timed. */
+ * form and block size on the V phase of this handle's own input, reps
+ * timed runs each after one warm-up, keep the fastest of each (minimum
+ * over runs) and set it.  best (may be NULL) receives the choice; ms (may
+ * be NULL, n_ms entries, up to 9) receives the times in the order STATIC,
+ * PERSISTENT, STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16; -1 for a
+ * candidate not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */

@@ -1285,6 +1285,148 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
   }
 }
 
+/* MFMA form of the Hadamard.  For one frequency and one neighbour
+ * direction o every parent applies the same 8 x 8 complex matrix
+ * T_o[c][c'] = T[f][2o + 3 + c' - c] (target child c, source child c') to
+ * its neighbour's 8 child spectra.  In real form [[Tr, -Ti], [Ti, Tr]]
+ * (16 x 16, rows: Re Y[c] then Im Y[c]; columns: Re X[c'] then Im X[c'])
+ * over 16 parents that is four 16x16x4 MFMAs, k = 4m + kk with source
+ * child (4m + kk) & 7, real parts for m < 2 and imaginary for m >= 2.
+ * Wave w owns parents 16w .. 16w + 15 of the group: lane j = lane & 15 is
+ * row i of A and parent column j of B, kk = lane >> 4.  Per direction a
+ * lane reads two translation values (A) and two neighbour spectra (B)
+ * from LDS; the group's neighbour-slot union and the frequency's 343
+ * translation values are staged per frequency, the next frequency's
+ * prefetched into registers meanwhile.  MFMA operand layouts as pinned by
+ * test_mfma_f64_operand_layout / test_mfma_f32_operand_layout. */
+__device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ v4f mfma4(float a, float b, v4f c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+template <typename R>
+struct Acc4;
+template <>
+struct Acc4<double> {
+  typedef v4d T;
+};
+template <>
+struct Acc4<float> {
+  typedef v4f T;
+};
+
+template <int NP, typename ST, int FB>
+__global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ groups,
+                                                       const int* __restrict__ uslots,
+                                                       const unsigned char* __restrict__ loc,
+                                                       const int* __restrict__ yslot, const ST* __restrict__ T,
+                                                       const ST* __restrict__ X, ST* __restrict__ Y) {
+  typedef typename Spec<ST>::R R;
+  typedef typename Acc4<R>::T V;
+  using G = Grid<NP>;
+  constexpr int F = G::F;
+  static_assert(F % FB == 0, "frequency blocks");
+  constexpr int PER = (HAD_CAP * 8 + 255) / 256;
+  constexpr int PERT = (343 + 255) / 256;
+  __shared__ ST Xs[HAD_CAP][9];
+  __shared__ ST Ts[343];
+  __shared__ unsigned char sloc[27][64];
+  const int4 g = groups[blockIdx.x];
+  const int f0 = blockIdx.y * FB;
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ne8 = g.w * 8;
+  int us[PER];
+#pragma unroll
+  for (int k = 0; k < PER; k++) {
+    const int e = tid + 256 * k;
+    us[k] = e < ne8 ? uslots[g.z + (e >> 3)] : 0;
+  }
+  ST pre[PER], pret[PERT];
+  auto fetch = [&](int f) {
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = tid + 256 * k;
+      pre[k] = e < ne8 ? X[((size_t)us[k] * F + f) * 8 + (e & 7)] : Spec<ST>::mk(0.0, 0.0);
+    }
+#pragma unroll
+    for (int k = 0; k < PERT; k++) {
+      const int e = tid + 256 * k;
+      pret[k] = e < 343 ? T[(size_t)f * 343 + e] : Spec<ST>::mk(0.0, 0.0);
+    }
+  };
+  fetch(f0);
+  /* union indices of the block's parents, [neighbour][parent]; parents past
+   * the group's count repeat its last parent */
+  for (int e = tid; e < 27 * 64; e += 256) {
+    const int l = e / 27, o = e - 27 * l;
+    sloc[o][l] = loc[(size_t)(g.x + min(l, g.y - 1)) * 27 + o];
+  }
+  const int j = lane & 15, kk = lane >> 4;
+  const int pj = 16 * wave + j;
+  const bool active = 16 * wave < g.y; /* wave-uniform */
+  /* row i = j of A: target child c = j & 7, real part for j < 8; the two
+   * source children this lane supplies are kk and kk + 4 */
+  const int c = j & 7;
+  const bool rrow = j < 8;
+  auto base = [&](int cs) {
+    return (3 + ((cs >> 2) & 1) - ((c >> 2) & 1)) + 7 * (3 + ((cs >> 1) & 1) - ((c >> 1) & 1)) +
+           49 * (3 + (cs & 1) - (c & 1));
+  };
+  const int b0 = base(kk), b1 = base(kk + 4);
+  const int ys = yslot[g.x + min(pj, g.y - 1)];
+  for (int fi = 0; fi < FB; fi++) {
+    const int f = f0 + fi;
+    __syncthreads(); /* the previous frequency's readers are done with Xs, Ts */
+#pragma unroll
+    for (int k = 0; k < PER; k++) {
+      const int e = tid + 256 * k;
+      if (e < ne8) Xs[e >> 3][e & 7] = pre[k];
+    }
+#pragma unroll
+    for (int k = 0; k < PERT; k++) {
+      const int e = tid + 256 * k;
+      if (e < 343) Ts[e] = pret[k];
+    }
+    __syncthreads();
+    if (fi + 1 < FB) fetch(f + 1);
+    if (!active) continue;
+    V accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+#pragma unroll 2
+    for (int o = 0; o < 27; o++) {
+      if (o == 13) continue; /* the parent itself: its children are all adjacent */
+      const int os = 2 * (o / 9 - 1) + 14 * ((o / 3) % 3 - 1) + 98 * (o % 3 - 1);
+      const ST t0 = Ts[b0 + os], t1 = Ts[b1 + os];
+      const ST* xs = Xs[sloc[o][pj]];
+      const ST x0 = xs[kk], x1 = xs[kk + 4];
+      accr = mfma4(rrow ? t0.x : t0.y, x0.x, accr);
+      accr = mfma4(rrow ? t1.x : t1.y, x1.x, accr);
+      acci = mfma4(rrow ? -t0.y : t0.x, x0.y, acci);
+      acci = mfma4(rrow ? -t1.y : t1.x, x1.y, acci);
+    }
+    const V acc = accr + acci;
+    ST* yb = Y + ((size_t)ys * F + f) * 8;
+    if constexpr (sizeof(R) == 8) {
+      /* D[i][j] in lane j + 16(i % 4), element i / 4: rows kk, kk+4 (Re)
+       * and kk+8, kk+12 (Im) = children kk and kk + 4 */
+      if (pj < g.y) {
+        yb[kk] = Spec<ST>::mk(acc[0], acc[2]);
+        yb[kk + 4] = Spec<ST>::mk(acc[1], acc[3]);
+      }
+    } else {
+      /* D[i][j] in lane j + 16(i / 4), element i % 4: lanes kk < 2 hold Re
+       * of children 4kk + e, lanes kk + 2 (the same column j) their Im */
+      R im[4];
+#pragma unroll
+      for (int e = 0; e < 4; e++) im[e] = __shfl_xor(acc[e], 32, 64);
+      if (pj < g.y && kk < 2)
+#pragma unroll
+        for (int e = 0; e < 4; e++) yb[4 * kk + e] = Spec<ST>::mk(acc[e], im[e]);
+    }
+  }
+}
+
 /* Inverse transform of the 8 sibling spectra of one child block, pruned to
  * the NP^3 cube holding the check surface; normalised by 1/N^3 and added
  * into the downward check values (compute_ifft_trg, fmm/omp_sse_block.cc:482-533:

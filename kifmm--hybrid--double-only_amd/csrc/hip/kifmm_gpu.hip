@@ -85,6 +85,7 @@ struct kifmm_gpu {
   /* launch shapes (kifmm_gpu_set_tuning / kifmm_gpu_autotune) */
   int p2p_form = KIFMM_P2P_STATIC;
   int had_fb = 8;
+  int had_form = KIFMM_HAD_MFMA;
   /* mixed path: per target, the U-list sources closer than MIX_CLOSE h */
   int* cptr = nullptr; int* cidx = nullptr;
   int close_listed = 0;
@@ -749,18 +750,23 @@ int vlist_phase(kifmm_gpu* g, ST* X, ST* Y, ST* T, hipEvent_t e1, hipEvent_t e2,
   LAUNCH_CHECK();
   REC(e2);
   if (g->n_hgroups > 0)
-    switch (g->had_fb) {
-      case 4:
-        hipLaunchKernelGGL((k_hadamard<NP, ST, 4>), dim3(g->n_hgroups, g->F / 4), dim3(128), 0, g->stream,
-                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
-        break;
-      case 16:
-        hipLaunchKernelGGL((k_hadamard<NP, ST, 16>), dim3(g->n_hgroups, g->F / 16), dim3(128), 0, g->stream,
-                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
-        break;
+    switch (g->had_fb * 2 + g->had_form) {
+#define KIFMM_HAD_CASE(FBV)                                                                                    \
+  case 2 * FBV:                                                                                                \
+    hipLaunchKernelGGL((k_hadamard<NP, ST, FBV>), dim3(g->n_hgroups, g->F / FBV), dim3(128), 0, g->stream,     \
+                       g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);                                   \
+    break;                                                                                                     \
+  case 2 * FBV + 1:                                                                                            \
+    hipLaunchKernelGGL((k_hadamard_mfma<NP, ST, FBV>), dim3(g->n_hgroups, g->F / FBV), dim3(256), 0, g->stream, \
+                       g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);                                   \
+    break;
+      KIFMM_HAD_CASE(4)
+      KIFMM_HAD_CASE(8)
+      KIFMM_HAD_CASE(16)
+#undef KIFMM_HAD_CASE
       default:
-        hipLaunchKernelGGL((k_hadamard<NP, ST, 8>), dim3(g->n_hgroups, g->F / 8), dim3(128), 0, g->stream,
-                           g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);
+        set_error("internal: unknown Hadamard shape");
+        return KIFMM_EINVAL;
     }
   LAUNCH_CHECK();
   REC(e3);
@@ -1163,17 +1169,21 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
     return KIFMM_EINVAL;
   }
   if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_STATIC64 ||
-      (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16)) {
-    set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value and hadamard_fb 4, 8 or 16");
+      (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
+      (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA)) {
+    set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16 and "
+              "hadamard_form a KIFMM_HAD_* value");
     return KIFMM_EINVAL;
   }
   if (t->p2p_form != KIFMM_P2P_STATIC && g->precision != KIFMM_PREC_FP64) {
     set_error("kifmm_gpu_set_tuning: the U-list forms apply to the FP64 path");
     return KIFMM_EUNSUP;
   }
-  if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb) return KIFMM_OK;
+  if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb && t->hadamard_form == g->had_form)
+    return KIFMM_OK;
   g->p2p_form = t->p2p_form;
   g->had_fb = t->hadamard_fb;
+  g->had_form = t->hadamard_form;
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   g->gexec = nullptr;
@@ -1189,6 +1199,7 @@ int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t) {
   std::memset(t, 0, sizeof *t);
   t->p2p_form = g->p2p_form;
   t->hadamard_fb = g->had_fb;
+  t->hadamard_form = g->had_form;
   return KIFMM_OK;
 }
 
@@ -1197,7 +1208,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[6] = {-1, -1, -1, -1, -1, -1};
+  double t[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
   auto time_phase = [&](unsigned phase, double* out) -> int {
@@ -1223,18 +1234,20 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   }
   c.p2p_form = best_form;
   const int fbs[3] = {4, 8, 16};
-  int best_fb = 0;
-  for (int k = 0; k < 3; k++) {
-    c.hadamard_fb = fbs[k];
+  int best_v = 0;
+  for (int k = 0; k < 6; k++) { /* VALU fb 4, 8, 16, then MFMA fb 4, 8, 16 */
+    c.hadamard_form = k / 3;
+    c.hadamard_fb = fbs[k % 3];
     TRY(kifmm_gpu_set_tuning(g, &c));
     TRY(time_phase(KIFMM_PHASE_V, &t[3 + k]));
-    if (t[3 + k] < t[3 + best_fb]) best_fb = k;
+    if (t[3 + k] < t[3 + best_v]) best_v = k;
   }
-  c.hadamard_fb = fbs[best_fb];
+  c.hadamard_form = best_v / 3;
+  c.hadamard_fb = fbs[best_v % 3];
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 6); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 9); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

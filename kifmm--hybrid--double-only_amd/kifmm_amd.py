@@ -70,10 +70,12 @@ class GpuOptions(C.Structure):
 
 
 class GpuTuning(C.Structure):
-    _fields_ = [("p2p_form", C.c_int), ("hadamard_fb", C.c_int), ("reserved", C.c_int * 6)]
+    _fields_ = [("p2p_form", C.c_int), ("hadamard_fb", C.c_int), ("hadamard_form", C.c_int),
+                ("reserved", C.c_int * 5)]
 
 
 P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64 = 0, 1, 2
+HAD_VALU, HAD_MFMA = 0, 1
 
 
 class PhaseTimes(C.Structure):
@@ -371,19 +373,21 @@ class GpuEvaluator:
     def tuning(self):
         t = GpuTuning()
         _check(lib.kifmm_gpu_get_tuning(self._h, C.byref(t)))
-        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb}
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form}
 
-    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8):
-        t = GpuTuning(p2p_form, hadamard_fb)
+    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8, hadamard_form=HAD_MFMA):
+        t = GpuTuning(p2p_form, hadamard_fb, hadamard_form)
         _check(lib.kifmm_gpu_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, reps=3):
         """Time the launch shapes on this input, keep the fastest; returns
-        (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, fb 4, 8, 16."""
+        (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, then the
+        Hadamard on the VALU with 4, 8, 16 frequencies per block and on MFMA
+        with 4, 8, 16."""
         t = GpuTuning()
-        ms = np.zeros(6)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 6))
-        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb}, ms
+        ms = np.zeros(9)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 9))
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form}, ms
 
     def work_counts(self):
         w = WorkCounts()

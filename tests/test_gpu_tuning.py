@@ -4,18 +4,28 @@ knobs, fmm/util.c:118-163).
 
 Every U-list form (one block per item of <= 128 or <= 64 targets, or the
 persistent per-XCD work queue) and every Hadamard frequency block (4, 8,
-16) computes each output in the same order, so the potentials must be
-bit-identical to the default shapes, eagerly and through a recaptured HIP
-graph, with separate and coincident targets.  The autotuner must return
-the fastest of the times it measured and leave that choice set."""
+16) computes each output in the same order, so for one Hadamard form the
+potentials must be bit-identical to the default shapes, eagerly and through
+a recaptured HIP graph, with separate and coincident targets.  The MFMA
+Hadamard sums the same terms in another order: its potentials must meet the
+FP64 parity bar against the oracle themselves, agree with the VALU form's
+within that bar (the summation-order floor of the pseudo-inverse applies,
+2.6e-13 at NP = 4) and within 1e-6 on the mixed path, and be bit-identical
+among its own shapes.  The autotuner must return the fastest
+of the times it measured and leave that choice set."""
 import numpy as np
 import pytest
 
 import kifmm_amd as K
+import oracle_lib as O
 
 pytestmark = pytest.mark.gpu
 
 SHAPES = [(f, fb) for f in (K.P2P_STATIC, K.P2P_PERSISTENT, K.P2P_STATIC64) for fb in (4, 8, 16)]
+
+
+def rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
 @pytest.mark.parametrize("n,dist,q,np_,c", [(60000, "uniform", 64, 4, 0), (50000, "ellipsoid", 32, 6, 0),
@@ -24,27 +34,47 @@ def test_every_shape_gives_identical_potentials(n, dist, q, np_, c):
     print(f"case n={n} dist={dist} q={q} NP={np_} C={c}", flush=True)
     prob = K.Problem.create(n, dist, q, np_=np_, seed=4, coincide=c)
     ev = K.GpuEvaluator(prob)
+    default, _ = ev.evaluate()
+    ev.set_tuning(K.P2P_STATIC, 8, K.HAD_VALU)
     ref, _ = ev.evaluate()
     gr = K.GpuEvaluator(prob, graph=True)
-    for f, fb in SHAPES:
-        ev.set_tuning(f, fb)
-        assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb}
-        got, _ = ev.evaluate()
-        assert np.array_equal(got, ref), (f, fb)
-        gr.set_tuning(f, fb)
-        for _ in range(2):
-            got, _ = gr.evaluate()
-            assert np.array_equal(got, ref), ("graph", f, fb)
+    ref_mfma = None
+    for form in (K.HAD_VALU, K.HAD_MFMA):
+        for f, fb in SHAPES:
+            ev.set_tuning(f, fb, form)
+            assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb, "hadamard_form": form}
+            got, _ = ev.evaluate()
+            if form == K.HAD_MFMA and ref_mfma is None:
+                ref_mfma = got
+                print(f"  MFMA vs VALU Hadamard: {rel(got, ref):.2e}", flush=True)
+                assert rel(got, ref) < O.PARITY_RTOL[np_]
+            want = ref if form == K.HAD_VALU else ref_mfma
+            assert np.array_equal(got, want), (form, f, fb)
+            if form == K.HAD_MFMA:
+                assert np.array_equal(got, default), ("default shapes", f, fb)
+            gr.set_tuning(f, fb, form)
+            for _ in range(2):
+                got, _ = gr.evaluate()
+                assert np.array_equal(got, want), ("graph", form, f, fb)
 
 
 def test_mixed_path_hadamard_blocks_and_forms():
     prob = K.Problem.create(40000, "uniform", 64, np_=4, seed=5)
     ev = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+    ev.set_tuning(K.P2P_STATIC, 8, K.HAD_VALU)
     ref, _ = ev.evaluate()
     for fb in (4, 16):
-        ev.set_tuning(K.P2P_STATIC, fb)
+        ev.set_tuning(K.P2P_STATIC, fb, K.HAD_VALU)
         got, _ = ev.evaluate()
         assert np.array_equal(got, ref), fb
+    ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA)
+    m8, _ = ev.evaluate()
+    print(f"  mixed MFMA vs VALU Hadamard: {rel(m8, ref):.2e}", flush=True)
+    assert rel(m8, ref) < 1e-6
+    for fb in (4, 16):
+        ev.set_tuning(K.P2P_STATIC, fb, K.HAD_MFMA)
+        got, _ = ev.evaluate()
+        assert np.array_equal(got, m8), fb
     with pytest.raises(K.KifmmError) as e:
         ev.set_tuning(K.P2P_PERSISTENT, 8)
     assert e.value.code == -4
@@ -52,11 +82,11 @@ def test_mixed_path_hadamard_blocks_and_forms():
 
 def test_invalid_tuning_is_rejected():
     ev = K.GpuEvaluator(K.Problem.create(5000, "uniform", 64, np_=4, seed=6))
-    for f, fb in ((3, 8), (-1, 8), (0, 5), (0, 0)):
+    for f, fb, form in ((3, 8, 0), (-1, 8, 0), (0, 5, 0), (0, 0, 0), (0, 8, 2), (0, 8, -1)):
         with pytest.raises(K.KifmmError) as e:
-            ev.set_tuning(f, fb)
+            ev.set_tuning(f, fb, form)
         assert e.value.code == -1
-    assert ev.tuning() == {"p2p_form": K.P2P_STATIC, "hadamard_fb": 8}
+    assert ev.tuning() == {"p2p_form": K.P2P_STATIC, "hadamard_fb": 8, "hadamard_form": K.HAD_MFMA}
 
 
 @pytest.mark.parametrize("precision", [K.PREC_FP64, K.PREC_MIXED])
@@ -69,7 +99,27 @@ def test_autotune_keeps_the_fastest(precision):
     nforms = 3 if precision == K.PREC_FP64 else 1
     assert np.all(ms[:nforms] > 0) and np.all(ms[nforms:3] == -1) and np.all(ms[3:] > 0)
     assert best["p2p_form"] == int(np.argmin(ms[:nforms]))
-    assert best["hadamard_fb"] == (4, 8, 16)[int(np.argmin(ms[3:]))]
+    v = int(np.argmin(ms[3:]))
+    assert (best["hadamard_form"], best["hadamard_fb"]) == (v // 3, (4, 8, 16)[v % 3])
     assert ev.tuning() == best
     got, _ = ev.evaluate()
-    assert np.array_equal(got, ref)
+    assert rel(got, ref) < (O.PARITY_RTOL[4] if precision == K.PREC_FP64 else 1e-6)
+
+
+@pytest.mark.parametrize("n,dist,q,np_,c", [(20000, "uniform", 32, 3, 0), (30000, "ellipsoid", 32, 4, 1),
+                                             (20000, "gaussmix", 64, 6, 0)])
+def test_mfma_hadamard_matches_oracle(n, dist, q, np_, c):
+    print(f"case n={n} dist={dist} q={q} NP={np_} C={c}", flush=True)
+    prob = K.Problem.create(n, dist, q, np_=np_, seed=9, coincide=c)
+    ev = K.GpuEvaluator(prob)
+    ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA)
+    got = prob.scatter(ev.evaluate()[0])
+    orc = O.Oracle(n, dist, q, np_, 9, c)
+    o = prob.operators()
+    orc.set_operators(*[np.ctypeslib.as_array(p, shape=(k,)).copy() for p, k in
+                        ((o.uc2ue, o.n_e * o.n_uc), (o.ue2uc, 8 * o.n_uc * o.n_e), (o.dc2de, o.n_e ** 2),
+                         (o.de2dc, 8 * o.n_e ** 2))])
+    ref, _ = orc.evaluate(O.KERNEL_EXACT)
+    r = rel(got, ref)
+    print(f"  MFMA Hadamard vs oracle {r:.2e}", flush=True)
+    assert r < O.PARITY_RTOL[np_]
