@@ -29,6 +29,9 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <strings.h>
 #include <vector>
@@ -536,8 +539,39 @@ void pinv_jacobi(const double* M, int m, int n, double eps, double* R) {
   for (size_t i = 0; i < acc.size(); i++) R[i] = (double)acc[i];
 }
 
+/* The four operator sets depend on NP only: built once per NP per process
+ * and copied into each problem that asks for them. */
+struct OperatorSet {
+  std::vector<double> uc2ue, ue2uc, dc2de, de2dc;
+};
+std::mutex g_ops_mutex;
+std::map<int, std::shared_ptr<const OperatorSet>> g_ops_cache;
+
+void compute_operators(kifmm_problem* p);
+
 void build_operators(kifmm_problem* p) {
   const double t0 = wall_seconds();
+  std::shared_ptr<const OperatorSet> hit;
+  {
+    std::lock_guard<std::mutex> lock(g_ops_mutex);
+    auto it = g_ops_cache.find(p->np);
+    if (it != g_ops_cache.end()) hit = it->second;
+  }
+  if (!hit) {
+    compute_operators(p);
+    auto set = std::make_shared<OperatorSet>();
+    set->uc2ue = p->uc2ue; set->ue2uc = p->ue2uc; set->dc2de = p->dc2de; set->de2dc = p->de2dc;
+    std::lock_guard<std::mutex> lock(g_ops_mutex);
+    g_ops_cache.emplace(p->np, std::move(set));
+  } else {
+    p->uc2ue = hit->uc2ue; p->ue2uc = hit->ue2uc; p->dc2de = hit->dc2de; p->de2dc = hit->de2dc;
+  }
+  p->have_ops = true;
+  p->t_ops = wall_seconds() - t0;
+}
+
+/* UC2UE, UE2UC, DC2DE, DE2DC for p->np (fmm/trans.cc:203-326) */
+void compute_operators(kifmm_problem* p) {
   const int np = p->np;
   const Surface ue = surface(np, 1.0), uc = surface(np + 2, 3.0), de = surface(np, 3.0), dc = surface(np, 1.0);
   const int ne = ue.n, nuc = uc.n;
@@ -578,8 +612,6 @@ void build_operators(kifmm_problem* p) {
       }
     }
   }
-  p->have_ops = true;
-  p->t_ops = wall_seconds() - t0;
 }
 
 kifmm_problem* checked_create(kifmm_problem* p) {

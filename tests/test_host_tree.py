@@ -211,3 +211,16 @@ def test_too_many_coincident_points_is_an_error():
         K.Problem.from_points(*pts, np.ones(50), pts_max=8, np_=4)
     t = K.Problem.from_points(*pts, np.ones(50), pts_max=50, np_=4).tree()
     assert t["n_nodes"] == 1 and t["src_num"][0] == 50
+
+
+def test_operators_are_shared_per_np():
+    """The operators depend on NP only and are built once per NP per
+    process: problems of any size and distribution get identical arrays,
+    and differ between NPs."""
+    a = K.Problem.create(3000, "uniform", 32, np_=3, seed=1)
+    b = K.Problem.create(5000, "ellipsoid", 16, np_=3, seed=2)
+    c = K.Problem.create(3000, "uniform", 32, np_=4, seed=1)
+    oa, ob, oc = a.operator_arrays(), b.operator_arrays(), c.operator_arrays()
+    for k in ("uc2ue", "ue2uc", "dc2de", "de2dc"):
+        assert np.array_equal(oa[k], ob[k]), k
+        assert oa[k].shape != oc[k].shape, k
