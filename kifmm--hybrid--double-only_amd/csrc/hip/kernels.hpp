@@ -759,59 +759,70 @@ static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restri
   }
 }
 
-static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items,
+/* lpb consecutive leaves per block in the mixed S2M and L2T (1 for leaves
+ * of ~100 points and more; 4 for small leaves, whose per-block cost and
+ * surface loads are then shared). */
+static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items, int n_items, int lpb,
                                                           const double4* __restrict__ geo,
                                                           const float* __restrict__ surf_uc32, int n_uc,
                                                           int ld_uc_surf, const float4* __restrict__ src32,
                                                           double* __restrict__ chk, int ld_chk) {
-  const int4 it = items[blockIdx.x];
   const int j = threadIdx.x, jj = min(j, n_uc - 1);
-  const double ih = 1.0 / geo[it.x].w;
   const float tx = surf_uc32[jj], ty = surf_uc32[ld_uc_surf + jj], tz = surf_uc32[2 * ld_uc_surf + jj];
-  const float4* __restrict__ src = src32 + it.y;
-  float acc0 = 0.0f, acc1 = 0.0f;
-  int k = 0;
-  for (; k + 2 <= it.z; k += 2) {
-    const float4 s0 = src[k], s1 = src[k + 1];
-    const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    const float bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
-    acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-    acc1 = __builtin_fmaf(s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+  for (int r = 0; r < lpb; r++) {
+    const int item = blockIdx.x * lpb + r;
+    if (item >= n_items) return;
+    const int4 it = items[item];
+    const double ih = 1.0 / geo[it.x].w;
+    const float4* __restrict__ src = src32 + it.y;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    int k = 0;
+    for (; k + 2 <= it.z; k += 2) {
+      const float4 s0 = src[k], s1 = src[k + 1];
+      const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+      const float bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
+      acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+      acc1 = __builtin_fmaf(s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+    }
+    if (k < it.z) {
+      const float4 s0 = src[k];
+      const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
+      acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    }
+    if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
   }
-  if (k < it.z) {
-    const float4 s0 = src[k];
-    const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-  }
-  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
 }
 
-static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items,
+static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items, int n_items, int lpb,
                                                          const double4* __restrict__ geo,
                                                          const float* __restrict__ surf_de, int ld_surf, int n_e,
                                                          const double* __restrict__ dden, int ld_e,
                                                          const double4* __restrict__ trg4, double* __restrict__ pot) {
-  const int4 it = items[blockIdx.x];
-  const int k = threadIdx.x;
-  const double4 q = trg4[it.y + min(k, it.z - 1)];
-  const double4 g = geo[it.x];
-  const double ih = 1.0 / g.w;
-  const float tx = (float)((q.x - g.x) * ih), ty = (float)((q.y - g.y) * ih), tz = (float)((q.z - g.z) * ih);
-  const double* __restrict__ den = dden + (size_t)it.x * ld_e;
-  float acc0 = 0.0f, acc1 = 0.0f;
-  int j = 0;
-  for (; j + 2 <= n_e; j += 2) {
-    const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-    const float bx = tx - surf_de[j + 1], by = ty - surf_de[ld_surf + j + 1],
-                bz = tz - surf_de[2 * ld_surf + j + 1];
-    acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-    acc1 = __builtin_fmaf((float)den[j + 1], __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+  for (int r = 0; r < lpb; r++) {
+    const int item = blockIdx.x * lpb + r;
+    if (item >= n_items) return;
+    const int4 it = items[item];
+    const int k = threadIdx.x;
+    const double4 q = trg4[it.y + min(k, it.z - 1)];
+    const double4 g = geo[it.x];
+    const double ih = 1.0 / g.w;
+    const float tx = (float)((q.x - g.x) * ih), ty = (float)((q.y - g.y) * ih), tz = (float)((q.z - g.z) * ih);
+    const double* __restrict__ den = dden + (size_t)it.x * ld_e;
+    float acc0 = 0.0f, acc1 = 0.0f;
+    int j = 0;
+    for (; j + 2 <= n_e; j += 2) {
+      const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
+      const float bx = tx - surf_de[j + 1], by = ty - surf_de[ld_surf + j + 1],
+                  bz = tz - surf_de[2 * ld_surf + j + 1];
+      acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+      acc1 = __builtin_fmaf((float)den[j + 1], __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+    }
+    if (j < n_e) {
+      const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
+      acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    }
+    if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((double)acc0 + (double)acc1);
   }
-  if (j < n_e) {
-    const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-    acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-  }
-  if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((double)acc0 + (double)acc1);
 }
 
 /* ------------------------------------------------------------ GEMM (MFMA) */

@@ -83,6 +83,7 @@ struct kifmm_gpu {
   int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
   int persist_grid = 0;
   /* launch shapes (kifmm_gpu_set_tuning / kifmm_gpu_autotune) */
+  int mix_lpb = 1; /* leaves per block of the mixed S2M / L2T */
   int p2p_form = KIFMM_P2P_STATIC;
   int had_fb = 8;
   int had_form = KIFMM_HAD_MFMA;
@@ -666,6 +667,20 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   return KIFMM_OK;
 }
 
+/* Mixed path, once per evaluator: leaves per block for the mixed S2M and
+ * L2T, and the sources in FP32 leaf-local units
+ * (positions and densities are fixed for the handle's lifetime) for
+ * k_s2m_mixed. */
+int mixed_setup(kifmm_gpu* g) {
+  g->mix_lpb = (g->n_s2m > 0 && (double)g->n_src / g->n_s2m < 100.0) ? 4 : 1;
+  if (g->n_s2m > 0)
+    hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
+                       g->src32);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return KIFMM_OK;
+}
+
 /* Close pairs of the mixed U-list (k_close_pairs): counts, offsets on the
  * host, then the indices.  An input whose list would exceed 16 pairs per
  * target on average keeps the in-loop test (k_p2p_ulist_mixed). */
@@ -839,12 +854,11 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
-    if (g->precision == KIFMM_PREC_MIXED) {
-      hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
-                         g->src32);
-      hipLaunchKernelGGL(k_s2m_mixed, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
-                         g->geo, g->surf_uc32, g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
-    } else
+    if (g->precision == KIFMM_PREC_MIXED) /* src32 was written at create time (mixed_setup) */
+      hipLaunchKernelGGL(k_s2m_mixed, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb), dim3(64 * ((g->n_uc + 63) / 64)),
+                         0, g->stream, g->s2m_items, g->n_s2m, g->mix_lpb, g->geo, g->surf_uc32, g->n_uc, g->ld_surf,
+                         g->src32, g->chk, lduc);
+    else
       hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
                          g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
     LAUNCH_CHECK();
@@ -886,7 +900,8 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
   {
     if (g->precision == KIFMM_PREC_MIXED)
-      hipLaunchKernelGGL(k_l2t_mixed, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de32,
+      hipLaunchKernelGGL(k_l2t_mixed, dim3((g->n_l2t + g->mix_lpb - 1) / g->mix_lpb), dim3(64), 0, g->stream,
+                         g->l2t_items, g->n_l2t, g->mix_lpb, g->geo, g->surf_de32,
                          g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
     else
       hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de, g->ld_surf,
@@ -1036,6 +1051,7 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
   }
   try {
     r = build(g, tree, ops);
+    if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED) r = mixed_setup(g);
     if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED && !(opts && (opts->flags & KIFMM_GPU_MIXED_INLINE)))
       r = close_list(g);
   } catch (const std::bad_alloc&) {
