@@ -759,18 +759,19 @@ static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restri
   }
 }
 
-/* lpb consecutive leaves per block in the mixed S2M and L2T (1 for leaves
+/* LPB consecutive leaves per block in the mixed S2M and L2T (1 for leaves
  * of ~100 points and more; 4 for small leaves, whose per-block cost and
  * surface loads are then shared). */
-static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items, int n_items, int lpb,
+template <int LPB>
+__global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items, int n_items,
                                                           const double4* __restrict__ geo,
                                                           const float* __restrict__ surf_uc32, int n_uc,
                                                           int ld_uc_surf, const float4* __restrict__ src32,
                                                           double* __restrict__ chk, int ld_chk) {
   const int j = threadIdx.x, jj = min(j, n_uc - 1);
   const float tx = surf_uc32[jj], ty = surf_uc32[ld_uc_surf + jj], tz = surf_uc32[2 * ld_uc_surf + jj];
-  for (int r = 0; r < lpb; r++) {
-    const int item = blockIdx.x * lpb + r;
+  for (int r = 0; r < LPB; r++) {
+    const int item = blockIdx.x * LPB + r;
     if (item >= n_items) return;
     const int4 it = items[item];
     const double ih = 1.0 / geo[it.x].w;
@@ -793,13 +794,14 @@ static __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict
   }
 }
 
-static __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items, int n_items, int lpb,
+template <int LPB>
+__global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items, int n_items,
                                                          const double4* __restrict__ geo,
                                                          const float* __restrict__ surf_de, int ld_surf, int n_e,
                                                          const double* __restrict__ dden, int ld_e,
                                                          const double4* __restrict__ trg4, double* __restrict__ pot) {
-  for (int r = 0; r < lpb; r++) {
-    const int item = blockIdx.x * lpb + r;
+  for (int r = 0; r < LPB; r++) {
+    const int item = blockIdx.x * LPB + r;
     if (item >= n_items) return;
     const int4 it = items[item];
     const int k = threadIdx.x;

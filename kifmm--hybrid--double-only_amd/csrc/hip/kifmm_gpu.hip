@@ -855,9 +855,9 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
     if (g->precision == KIFMM_PREC_MIXED) /* src32 was written at create time (mixed_setup) */
-      hipLaunchKernelGGL(k_s2m_mixed, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb), dim3(64 * ((g->n_uc + 63) / 64)),
-                         0, g->stream, g->s2m_items, g->n_s2m, g->mix_lpb, g->geo, g->surf_uc32, g->n_uc, g->ld_surf,
-                         g->src32, g->chk, lduc);
+      hipLaunchKernelGGL(g->mix_lpb == 4 ? k_s2m_mixed<4> : k_s2m_mixed<1>, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb),
+                         dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->n_s2m, g->geo, g->surf_uc32,
+                         g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
     else
       hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
                          g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
@@ -900,8 +900,8 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
   {
     if (g->precision == KIFMM_PREC_MIXED)
-      hipLaunchKernelGGL(k_l2t_mixed, dim3((g->n_l2t + g->mix_lpb - 1) / g->mix_lpb), dim3(64), 0, g->stream,
-                         g->l2t_items, g->n_l2t, g->mix_lpb, g->geo, g->surf_de32,
+      hipLaunchKernelGGL(g->mix_lpb == 4 ? k_l2t_mixed<4> : k_l2t_mixed<1>, dim3((g->n_l2t + g->mix_lpb - 1) / g->mix_lpb),
+                         dim3(64), 0, g->stream, g->l2t_items, g->n_l2t, g->geo, g->surf_de32,
                          g->ld_surf, g->n_e, g->dden, lde, g->trg4, g->pot);
     else
       hipLaunchKernelGGL(k_l2t, dim3(g->n_l2t), dim3(64), 0, g->stream, g->l2t_items, g->geo, g->surf_de, g->ld_surf,
