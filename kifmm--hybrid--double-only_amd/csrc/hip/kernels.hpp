@@ -142,6 +142,41 @@ constexpr int P2P_TILE = 512;
 constexpr int P2P_G = 4;
 constexpr int P2P_MAXT = 4 * 8 * P2P_G;
 
+/* Walks an item's source ranges {beg, num} one tile (P2P_TILE sources) at a
+ * time, in order.  The walk state is block-uniform; thread tid fetches the
+ * sources of tile slots tid and tid + 256 into registers, so a tile's loads
+ * are in flight while the previous tile is computed.  Returns the tile's
+ * source count (0 when the ranges are used up). */
+constexpr int P2P_PER = P2P_TILE / 256;
+struct RangeWalk {
+  int ri, ro, rlast;
+};
+__device__ __forceinline__ int walk_tile(RangeWalk& w, const int2* __restrict__ rng,
+                                         const double4* __restrict__ src4, int tid, double4 (&pre)[P2P_PER]) {
+  int fill = 0, idx[P2P_PER];
+#pragma unroll
+  for (int m = 0; m < P2P_PER; m++) idx[m] = -1;
+  while (fill < P2P_TILE && w.ri < w.rlast) {
+    const int2 r = rng[w.ri];
+    const int take = min(r.y - w.ro, P2P_TILE - fill);
+#pragma unroll
+    for (int m = 0; m < P2P_PER; m++) {
+      const int sl = tid + 256 * m - fill;
+      if (sl >= 0 && sl < take) idx[m] = r.x + w.ro + sl;
+    }
+    fill += take;
+    w.ro += take;
+    if (w.ro == r.y) {
+      w.ri++;
+      w.ro = 0;
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < P2P_PER; m++)
+    if (idx[m] >= 0) pre[m] = src4[idx[m]];
+  return fill;
+}
+
 template <bool MASK, int G>
 __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, const int* __restrict__ rng_ptr,
                                          const int2* __restrict__ rng, const double4* __restrict__ src4,
@@ -161,7 +196,9 @@ __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, 
   const int rlast = rng_ptr[it.x + 1];
   while (ri < rlast) {
     /* fill the tile: block-uniform walk over the ranges (every range has
-     * num >= 1, so each pass consumes at least one source) */
+     * num >= 1, so each pass consumes at least one source).  Loading the
+     * next tile ahead into registers (as the mixed kernel does) costs this
+     * kernel spills and a wave of occupancy: measured slower. */
     __syncthreads();
     int fill = 0;
     while (fill < P2P_TILE && ri < rlast) {
@@ -566,25 +603,19 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
       acc[q][e] = 0.0;
     }
   }
-  int ri = rng_ptr[it.x], ro = 0;
-  const int rlast = rng_ptr[it.x + 1];
-  while (ri < rlast) {
-    __syncthreads();
-    int fill = 0;
-    while (fill < P2P_TILE && ri < rlast) {
-      const int2 r = rng[ri];
-      const int take = min(r.y - ro, P2P_TILE - fill);
-      for (int k = tid; k < take; k += 256) {
-        const double4 q = src4[r.x + ro + k];
+  RangeWalk walk{rng_ptr[it.x], 0, rng_ptr[it.x + 1]};
+  double4 pre[P2P_PER];
+  int fill = walk_tile(walk, rng, src4, tid, pre);
+  while (fill > 0) {
+    __syncthreads(); /* the previous tile's readers are done */
+#pragma unroll
+    for (int m = 0; m < P2P_PER; m++) {
+      const int k = tid + 256 * m;
+      if (k < fill) {
+        const double4 q = pre[m];
         const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
-        gram_b_rows(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)), sB, fill + k);
-        sW[fill + k] = (float)q.w;
-      }
-      fill += take;
-      ro += take;
-      if (ro == r.y) {
-        ri++;
-        ro = 0;
+        gram_b_rows(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)), sB, k);
+        sW[k] = (float)q.w;
       }
     }
     const int ns = (fill + 31) & ~31; /* whole chunk pairs (P2P_TILE is a multiple of 32) */
@@ -593,8 +624,10 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
       sW[k] = 0.0f;
     }
     __syncthreads();
+    const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
     if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, ns >> 4, a, c, acc, lane);
     else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, ns >> 4, a, c, acc, lane);
+    fill = next;
   }
   const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
 #pragma unroll
