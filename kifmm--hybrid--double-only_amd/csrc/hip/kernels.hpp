@@ -1658,19 +1658,24 @@ static __global__ __launch_bounds__(64) void k_l2t(const int4* __restrict__ item
   const double4 g = geo[it.x];
   const double ih = 1.0 / g.w;
   const double tx = (q.x - g.x) * ih, ty = (q.y - g.y) * ih, tz = (q.z - g.z) * ih;
+  const double ax = -2.0 * tx, ay = -2.0 * ty, az = -2.0 * tz, tt = __builtin_fma(tx, tx, __builtin_fma(ty, ty, tz * tz));
   const double* __restrict__ den = dden + (size_t)it.x * ld_e;
+  /* Gram form (surface points >= 1.27 leaf units from the leaf's targets):
+   * r^2 = |t|^2 + |p|^2 - 2 t.p, |p|^2 from the surface table's row 3 */
   double acc[4] = {0.0, 0.0, 0.0, 0.0};
   int j = 0;
   for (; j + 4 <= n_e; j += 4) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
-      const double ax = tx - surf_de[j + u], ay = ty - surf_de[ld_surf + j + u], az = tz - surf_de[2 * ld_surf + j + u];
-      acc[u] = __builtin_fma(den[j + u], inv_r2<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
+      const double r2 = __builtin_fma(surf_de[j + u], ax, __builtin_fma(surf_de[ld_surf + j + u], ay,
+                                      __builtin_fma(surf_de[2 * ld_surf + j + u], az, tt + surf_de[3 * ld_surf + j + u])));
+      acc[u] = __builtin_fma(den[j + u], inv_r2<false>(r2), acc[u]);
     }
   }
   for (; j < n_e; j++) {
-    const double ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-    acc[0] = __builtin_fma(den[j], inv_r2<false>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
+    const double r2 = __builtin_fma(surf_de[j], ax, __builtin_fma(surf_de[ld_surf + j], ay,
+                                    __builtin_fma(surf_de[2 * ld_surf + j], az, tt + surf_de[3 * ld_surf + j])));
+    acc[0] = __builtin_fma(den[j], inv_r2<false>(r2), acc[0]);
   }
   if (k < it.z) pot[it.y + k] += HALF_OOFP_R * ih * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }

@@ -137,8 +137,12 @@ int dev_upload(kifmm_gpu* g, T** dst, const T* src, size_t n) {
   }
   g->allocs.push_back(p);
   *dst = (T*)p;
-  if (src && n) HIP_TRY(hipMemcpy(p, src, n * sizeof(T), hipMemcpyHostToDevice));
-  else HIP_TRY(hipMemset(p, 0, bytes));
+  /* on the handle's (non-blocking) stream, so that the kernels that fill
+   * the buffer at create time run after the clear: a null-stream memset is
+   * not ordered with them */
+  if (src && n) HIP_TRY(hipMemcpyAsync(p, src, n * sizeof(T), hipMemcpyHostToDevice, g->stream));
+  else HIP_TRY(hipMemsetAsync(p, 0, bytes, g->stream));
+  if (src && n) HIP_TRY(hipStreamSynchronize(g->stream)); /* src may be a temporary */
   return KIFMM_OK;
 }
 
@@ -153,7 +157,8 @@ int dev_zero(kifmm_gpu* g, T** dst, size_t n) {
     if (r_ != KIFMM_OK) return r_; \
   } while (0)
 
-/* boundary nodes of an np^3 grid on [-R,R]^3, x slowest (fmm/trans.cc:91-120) */
+/* boundary nodes of an np^3 grid on [-R,R]^3, x slowest (fmm/trans.cc:91-120):
+ * rows x, y, z, then |p|^2 (row 3, for the Gram-form L2T) */
 void host_surface(int np, double R, std::vector<double>& xyz, int ld) {
   const double step = 2.0 / (np - 1);
   int cnt = 0;
@@ -161,9 +166,11 @@ void host_surface(int np, double R, std::vector<double>& xyz, int ld) {
     for (int j = 0; j < np; j++)
       for (int k = 0; k < np; k++) {
         if (!(i == 0 || i == np - 1 || j == 0 || j == np - 1 || k == 0 || k == np - 1)) continue;
-        xyz[cnt] = R * (-1.0 + i * step);
-        xyz[ld + cnt] = R * (-1.0 + j * step);
-        xyz[2 * ld + cnt] = R * (-1.0 + k * step);
+        const double x = R * (-1.0 + i * step), y = R * (-1.0 + j * step), z = R * (-1.0 + k * step);
+        xyz[cnt] = x;
+        xyz[ld + cnt] = y;
+        xyz[2 * ld + cnt] = z;
+        xyz[3 * ld + cnt] = x * x + y * y + z * z;
         cnt++;
       }
 }
@@ -576,7 +583,7 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   /* unit surfaces (fmm/trans.cc:73-86) and DFT tables */
   {
     const int ls = g->ld_surf;
-    std::vector<double> s(3 * (size_t)ls, 0.0);
+    std::vector<double> s(4 * (size_t)ls, 0.0);
     host_surface(np, 1.0, s, ls); TRY(dev_upload(g, &g->surf_ue, s.data(), s.size()));
     std::fill(s.begin(), s.end(), 0.0);
     host_surface(np + 2, 3.0, s, ls); TRY(dev_upload(g, &g->surf_uc, s.data(), s.size()));
