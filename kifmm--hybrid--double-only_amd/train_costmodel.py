@@ -50,13 +50,13 @@ def draw_inputs(count: int, seed0: int, rng_seed: int):
 def sweep(inp, qs, steps, tag=""):
     """Every (q, path) of one input: features and measured phase times."""
     n, dist, np_, seed = inp
-    base = K.Problem.create(n, dist, 64, np_=np_, seed=seed)
+    base = K.Problem.create(n, dist, 64, np_=np_, seed=seed, device=0)
     pts = base.points()
     pts = (pts["sx"], pts["sy"], pts["sz"], pts["den"], pts["tx"], pts["ty"], pts["tz"])
     ops = base.operators()
     rows = []
     for q in qs:
-        prob = K.Problem.from_points(*pts, pts_max=q, np_=np_)
+        prob = K.Problem.from_points(*pts, pts_max=q, np_=np_, device=0)  # same tree as the host build
         for prec in CM.paths_for(np_):
             f = CM.features(prob, q, prec, ops)
             ph, wall = CM.phase_times(prob, prec, steps=steps)
