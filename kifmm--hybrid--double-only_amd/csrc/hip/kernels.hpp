@@ -629,6 +629,45 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, ns >> 4, a, c, acc, lane);
     fill = next;
   }
+  /* FP64 close-pair terms of the wave's target groups.  A group's pairs are
+   * one contiguous run of cidx (targets in order), spread over the 64 lanes
+   * whatever their distribution over the 16 targets, then summed per target
+   * by a segmented scan over the lanes (fixed order: deterministic). */
+  __shared__ double sC[P2P_MAXT];
+#pragma unroll
+  for (int q = 0; q < 2; q++) {
+    const int grp = wave + 4 * q;
+    if (grp < ngroups) {
+      const int t0 = 16 * grp, nt = min(16, it.z - t0), tg0 = it.y + t0;
+      if (lane < 16) sC[t0 + lane] = 0.0;
+      const int kbeg = cptr[tg0], kend = cptr[tg0 + nt];
+      const int ej = cptr[tg0 + min(lane & 15, nt - 1) + 1]; /* end of target lane&15's run */
+      for (int kb = kbeg; kb < kend; kb += 64) {
+        const int k = kb + lane;
+        const bool live = k < kend;
+        int tl = 0; /* the target of pair k: the number of runs that end at or before k */
+#pragma unroll
+        for (int j = 0; j < 16; j++) tl += __builtin_amdgcn_readlane(ej, j) <= k;
+        tl = live ? tl : 16;
+        double v = 0.0;
+        if (live) {
+          const double4 sp = src4[cidx[k]];
+          const double4 tp = sT[t0 + tl];
+          const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
+          const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+          v = sp.w * (inv_r<MASK>(r2) - 1.0);
+        }
+#pragma unroll
+        for (int off = 1; off < 64; off <<= 1) {
+          const double u = __shfl_up(v, off, 64);
+          const int ut = __shfl_up(tl, off, 64);
+          if (lane >= off && ut == tl) v += u;
+        }
+        const int nx = __shfl_down(tl, 1, 64);
+        if (live && (lane == 63 || nx != tl)) sC[t0 + tl] += v; /* one writer per target and chunk */
+      }
+    }
+  }
   const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
 #pragma unroll
   for (int q = 0; q < 2; q++) {
@@ -642,18 +681,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
         v += __shfl_xor(v, 2, 64);
         v += __shfl_xor(v, 1, 64);
         const int t = 16 * grp + 4 * (lane >> 4) + e;
-        if ((lane & 15) == 0 && t < it.z) {
-          const int tg = it.y + t;
-          const double4 tp = sT[t];
-          double corr = 0.0;
-          for (int k = cptr[tg]; k < cptr[tg + 1]; k++) {
-            const double4 sp = src4[cidx[k]];
-            const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
-            const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-            corr = __builtin_fma(sp.w, inv_r<MASK>(r2) - 1.0, corr);
-          }
-          pot[tg] = scale * (v + corr);
-        }
+        if ((lane & 15) == 0 && t < it.z) pot[it.y + t] = scale * (v + sC[t]);
       }
     }
   }
