@@ -11,14 +11,15 @@
  *   k_trans ........... compute_fft_trans__gpu_eval + D2Z x343  (874-963)
  *   k_hadamard ........ vlist_calc__gpu_eval                    (1200-1240)
  *   k_dft_inv ......... scale_nrmfc + cuFFT Z2D + regVal2SamVal (965-1196)
- *   k_wlist ........... wlist_eval__gpu                         (2070-2197)
- *   k_xlist ........... xlist_eval__gpu                         (2380-2493)
+ *   k_wx<., false> .... wlist_eval__gpu                         (2070-2197)
+ *   k_wx<., true> ..... xlist_eval__gpu                         (2380-2493)
  *   k_l2t ............. down_eval__gpu_leaves                   (1804-1872)
  *
- * Point evaluations share one scheme: a wave owns 8 targets held in
- * scalar registers, its 64 lanes stride over a source tile staged in LDS,
- * and the 8 per-lane partial sums are combined by a 6-step butterfly
- * (recursive halving) across the wave.  Sources are (x, y, z, w) double4
+ * The FP64 U-list gives a wave 8 targets held in scalar registers, its 64
+ * lanes stride over a source tile staged in LDS, and the 8 per-lane partial
+ * sums are combined by a 6-step butterfly (recursive halving) across the
+ * wave.  The W / X lists and the L2T give each lane a target and read the
+ * sources as wave-wide broadcasts.  Sources are (x, y, z, w) double4
  * records; tiles are padded with zero-weight points far outside the root
  * box so no per-pair bounds checks are needed.
  */
@@ -1575,100 +1576,150 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
 
 /* ------------------------------------------------- W, X and L2T evaluation */
 
-/* Adds 1/4pi * (field of the staged tile) to `ntrg` target points of a
- * leaf (trg4 + tbeg, out = pot + tbeg) or of a surface (surf != nullptr:
- * c + r * SP, out = check values). */
-template <bool MASK>
-__device__ __forceinline__ void apply_tile(const double4* tile, int ns, const double4* __restrict__ trg4, int tbeg,
-                                           int ntrg, const double* __restrict__ surf, int ld_surf, double4 g,
-                                           double* __restrict__ out, int lane, int wave, int nwaves) {
-  const int ngroups = (ntrg + 7) / 8;
-  for (int grp = wave; grp < ngroups; grp += nwaves) {
-    double tx[8], ty[8], tz[8], acc[8];
+/* W- and X-list evaluation (wlist_eval__gpu, fmm/cuda.cu:2070-2197;
+ * xlist_eval__gpu, fmm/cuda.cu:2380-2493): one wave per item, a lane per
+ * target point, the point in registers for the whole item and its sum in
+ * four lane-private accumulators (no cross-lane reduction).  Item = two int4:
+ *   a = {node t, first point, count <= 64, kind}: kind 0 the targets of
+ *       leaf t (trg4 from tbeg[t]), kind 1 the DC check surface c + r SP of
+ *       node t, results into dchk (X-list nodes with n_e or more targets);
+ *   b = {first entry, end entry, partial slot or -1, 0}: a chunk of t's
+ *       list (lidx), cut at entry boundaries.
+ * The sources of entry s are leaf points (every X-list entry; W-list leaves
+ * with fewer than n_e sources) or the UE surface c_s + r_s SP with the
+ * densities uden[s] (other W-list nodes), staged WX_TILE at a time in LDS
+ * and read as wave-wide broadcasts, the next tile loaded into registers
+ * while the current one is summed.  A list longer than one chunk is split:
+ * its chunks write their sums to part[slot][lane] and k_wx_reduce adds them
+ * to the output in chunk order (deterministic). */
+constexpr int WX_TILE = 256;
+constexpr int WX_PER = WX_TILE / 64;
+constexpr int WX_CHUNK_SRC = 1024; /* sources per chunk (an entry is never split) */
+
+struct WxWalk {
+  int k, kend, base; /* entry, end of the chunk, offset within the entry */
+};
+
+/* Loads the next tile (up to WX_TILE sources of one entry, padded with
+ * zero-weight far points) into registers; returns its count, 0 at the end. */
+template <bool XL>
+__device__ __forceinline__ int wx_fetch(WxWalk& w, const int* __restrict__ lidx, const int* __restrict__ child,
+                                        const int* __restrict__ snum, const int* __restrict__ sbeg,
+                                        const double4* __restrict__ geo, const double* __restrict__ surf_ue,
+                                        int ld_surf, int n_e, const double* __restrict__ uden, int ld_e,
+                                        const double4* __restrict__ src4, int lane, double4 (&pre)[WX_PER]) {
+  int s = 0, n = 0;
+  bool direct = true;
+  for (; w.k < w.kend; w.k++, w.base = 0) { /* skips entries without sources */
+    s = lidx[w.k];
+    direct = XL || (child[s] == -1 && snum[s] < n_e);
+    n = direct ? snum[s] : n_e;
+    if (w.base < n) break;
+  }
+  if (w.k >= w.kend) return 0;
+  const int cnt = min(WX_TILE, n - w.base);
+  if (direct) {
+    const double4* __restrict__ p = src4 + sbeg[s] + w.base;
 #pragma unroll
-    for (int t = 0; t < 8; t++) {
-      const int j = min(grp * 8 + t, ntrg - 1);
-      if (surf) {
-        tx[t] = g.x + g.w * surf[j];
-        ty[t] = g.y + g.w * surf[ld_surf + j];
-        tz[t] = g.z + g.w * surf[2 * ld_surf + j];
-      } else {
-        const double4 q = trg4[tbeg + j];
-        tx[t] = q.x; ty[t] = q.y; tz[t] = q.z;
-      }
-      acc[t] = 0.0;
+    for (int m = 0; m < WX_PER; m++) {
+      const int i = lane + 64 * m;
+      pre[m] = i < cnt ? p[i] : make_double4(FAR, FAR, FAR, 0.0);
     }
-    eval8_lds<MASK>(tile, ns, tx, ty, tz, acc, lane);
-    const double sum = wave_reduce8(acc, lane);
-    const int j = grp * 8 + (lane >> 3);
-    if ((lane & 7) == 0 && j < ntrg) out[j] += HALF_OOFP_R * sum;
+  } else {
+    const double4 g = geo[s];
+    const double* __restrict__ d = uden + (size_t)s * ld_e + w.base;
+#pragma unroll
+    for (int m = 0; m < WX_PER; m++) {
+      const int i = lane + 64 * m, j = w.base + i;
+      pre[m] = i < cnt ? make_double4(g.x + g.w * surf_ue[j], g.y + g.w * surf_ue[ld_surf + j],
+                                      g.z + g.w * surf_ue[2 * ld_surf + j], d[i])
+                       : make_double4(FAR, FAR, FAR, 0.0);
+    }
   }
+  w.base += cnt;
+  if (w.base >= n) {
+    w.k++;
+    w.base = 0;
+  }
+  return cnt;
 }
 
-/* W-list (wlist_eval__gpu, fmm/cuda.cu:2070-2197): item = {target leaf,
- * first target, target count <= WX_CHUNK}; a leaf with many targets is
- * spread over several blocks. */
-constexpr int WX_CHUNK = 256;
+/* acc[u] += 2 w / |x - p| over the staged sources [0, n4), n4 a multiple of 4 */
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_wlist(const int4* __restrict__ items, const int* __restrict__ wptr,
-                                               const int* __restrict__ widx, const int* __restrict__ child,
-                                               const int* __restrict__ snum, const int* __restrict__ sbeg,
-                                               const int* __restrict__ tbeg, const int* __restrict__ tnum,
-                                               const double4* __restrict__ geo, const double* __restrict__ surf_ue,
-                                               int ld_surf, int n_e, const double* __restrict__ uden, int ld_e,
-                                               const double4* __restrict__ src4, const double4* __restrict__ trg4,
-                                               double* __restrict__ pot) {
-  __shared__ double4 tile[1024];
-  const int4 it = items[blockIdx.x];
-  const int t = it.x, tb = tbeg[t] + it.y;
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  for (int k = wptr[t]; k < wptr[t + 1]; k++) {
-    const int s = widx[k];
-    const bool direct = child[s] == -1 && snum[s] < n_e;
-    const int n = direct ? snum[s] : n_e;
-    for (int base = 0; base < n; base += 1024) {
-      __syncthreads();
-      const int ns = direct ? stage_sources(tile, 1024, src4, sbeg[s] + base, n - base, nullptr, 0, geo[s], nullptr, tid, 256)
-                            : stage_sources(tile, 1024, nullptr, 0, n, surf_ue, ld_surf, geo[s], uden + (size_t)s * ld_e, tid, 256);
-      __syncthreads();
-      apply_tile<MASK>(tile, ns, trg4, tb, it.z, nullptr, 0, geo[t], pot + tb, lane, wave, 4);
+__device__ __forceinline__ void wx_sum_tile(const double4* tile, int n4, double tx, double ty, double tz,
+                                            double (&acc)[4]) {
+  for (int j = 0; j < n4; j += 4) {
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+      const double4 p = tile[j + u];
+      const double dx = tx - p.x, dy = ty - p.y, dz = tz - p.z;
+      const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      acc[u] = __builtin_fma(p.w, inv_r2<MASK>(r2), acc[u]);
     }
   }
 }
 
-/* X-list (xlist_eval__gpu, fmm/cuda.cu:2380-2493): item = {target node,
- * first target, target count <= WX_CHUNK, direct}; direct items (a leaf
- * with fewer than n_e targets) evaluate at their targets, the others at the
- * node's DC check surface (one item per node). */
-template <bool MASK>
-__global__ __launch_bounds__(256) void k_xlist(const int4* __restrict__ items, const int* __restrict__ xptr,
-                                               const int* __restrict__ xidx, const int* __restrict__ child,
-                                               const int* __restrict__ snum, const int* __restrict__ sbeg,
-                                               const int* __restrict__ tbeg, const int* __restrict__ tnum,
-                                               const double4* __restrict__ geo, const double* __restrict__ surf_dc,
-                                               int ld_surf, int n_e, double* __restrict__ dchk, int ld_e,
-                                               const double4* __restrict__ src4, const double4* __restrict__ trg4,
-                                               double* __restrict__ pot) {
-  __shared__ double4 tile[1024];
-  const int4 it = items[blockIdx.x];
-  const int t = it.x, tb = tbeg[t] + it.y;
-  const int tid = threadIdx.x, lane = tid & 63, wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const bool direct = it.w != 0;
-  for (int k = xptr[t]; k < xptr[t + 1]; k++) {
-    const int s = xidx[k];
-    for (int base = 0; base < snum[s]; base += 1024) {
-      __syncthreads();
-      const int ns = stage_sources(tile, 1024, src4, sbeg[s] + base, snum[s] - base, nullptr, 0, geo[s], nullptr, tid, 256);
-      __syncthreads();
-      if (direct)
-        apply_tile<MASK>(tile, ns, trg4, tb, it.z, nullptr, 0, geo[t], pot + tb, lane, wave, 4);
-      else
-        apply_tile<false>(tile, ns, nullptr, 0, n_e, surf_dc, ld_surf, geo[t], dchk + (size_t)t * ld_e, lane, wave, 4);
-    }
+/* XL = false: W-list (lidx = w_idx), XL = true: X-list (lidx = x_idx) */
+template <bool MASK, bool XL>
+__global__ __launch_bounds__(64) void k_wx(const int4* __restrict__ items, const int* __restrict__ lidx,
+                                           const int* __restrict__ child, const int* __restrict__ snum,
+                                           const int* __restrict__ sbeg, const int* __restrict__ tbeg,
+                                           const double4* __restrict__ geo, const double* __restrict__ surf_ue,
+                                           const double* __restrict__ surf_dc, int ld_surf, int n_e,
+                                           const double* __restrict__ uden, int ld_e,
+                                           const double4* __restrict__ src4, const double4* __restrict__ trg4,
+                                           double* __restrict__ pot, double* __restrict__ dchk,
+                                           double* __restrict__ part) {
+  __shared__ double4 tile[WX_TILE];
+  const int4 a = items[2 * blockIdx.x], b = items[2 * blockIdx.x + 1];
+  const int lane = threadIdx.x;
+  const int pt = a.y + min(lane, a.z - 1);
+  double tx, ty, tz;
+  if (a.w) {
+    const double4 g = geo[a.x];
+    tx = g.x + g.w * surf_dc[pt];
+    ty = g.y + g.w * surf_dc[ld_surf + pt];
+    tz = g.z + g.w * surf_dc[2 * ld_surf + pt];
+  } else {
+    const double4 q = trg4[tbeg[a.x] + pt];
+    tx = q.x; ty = q.y; tz = q.z;
+  }
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  WxWalk w{b.x, b.y, 0};
+  double4 pre[WX_PER];
+  int cnt = wx_fetch<XL>(w, lidx, child, snum, sbeg, geo, surf_ue, ld_surf, n_e, uden, ld_e, src4, lane, pre);
+  while (cnt > 0) {
+    __syncthreads(); /* the previous tile's reads are done (one wave: an LDS ordering point) */
+#pragma unroll
+    for (int m = 0; m < WX_PER; m++) tile[lane + 64 * m] = pre[m];
+    __syncthreads();
+    const int next = wx_fetch<XL>(w, lidx, child, snum, sbeg, geo, surf_ue, ld_surf, n_e, uden, ld_e, src4, lane, pre);
+    if (a.w) wx_sum_tile<false>(tile, (cnt + 3) & ~3, tx, ty, tz, acc);
+    else wx_sum_tile<MASK>(tile, (cnt + 3) & ~3, tx, ty, tz, acc);
+    cnt = next;
+  }
+  const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  if (lane < a.z) {
+    if (b.z >= 0) part[(size_t)b.z * 64 + lane] = sum;
+    else if (a.w) dchk[(size_t)a.x * ld_e + a.y + lane] += HALF_OOFP_R * sum;
+    else pot[tbeg[a.x] + a.y + lane] += HALF_OOFP_R * sum;
   }
 }
 
-/* L2T (down_eval__gpu_leaves, fmm/cuda.cu:1804-1872): one block per target leaf. */
+/* Adds the chunk sums of a split W / X item to its output, in chunk order.
+ * Item = {node t, first point, count, kind} and {first slot, slots, 0, 0}. */
+static __global__ __launch_bounds__(64) void k_wx_reduce(const int4* __restrict__ items,
+                                                        const int* __restrict__ tbeg, int ld_e,
+                                                        const double* __restrict__ part,
+                                                        double* __restrict__ pot, double* __restrict__ dchk) {
+  const int4 a = items[2 * blockIdx.x], b = items[2 * blockIdx.x + 1];
+  const int lane = threadIdx.x;
+  if (lane >= a.z) return;
+  double s = 0.0;
+  for (int c = 0; c < b.y; c++) s += part[(size_t)(b.x + c) * 64 + lane];
+  if (a.w) dchk[(size_t)a.x * ld_e + a.y + lane] += HALF_OOFP_R * s;
+  else pot[tbeg[a.x] + a.y + lane] += HALF_OOFP_R * s;
+}
 /* L2T (down_eval__gpu_leaves, fmm/cuda.cu:1804-1872): item = {leaf, first
  * target, count <= 64}, one wave; lane = target, the leaf's DE surface
  * points c + r * SP_DE[j] with densities dden[leaf][j] are wave-uniform. */

@@ -23,6 +23,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -74,8 +75,8 @@ struct kifmm_gpu {
   float *surf_uc32 = nullptr, *surf_de32 = nullptr;
   float4* src32 = nullptr;
   int *cube2surf = nullptr, *surf2cube = nullptr;
-  int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *tnum = nullptr, *depth = nullptr;
-  int *wptr = nullptr, *widx = nullptr, *xptr = nullptr, *xidx = nullptr;
+  int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *depth = nullptr;
+  int *widx = nullptr, *xidx = nullptr;
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
@@ -99,8 +100,11 @@ struct kifmm_gpu {
   int4* hgroups = nullptr; int n_hgroups = 0;
   int* huslots = nullptr;
   unsigned char* hloc = nullptr;
-  int4* witems = nullptr; int n_witems = 0;
+  int4* witems = nullptr; int n_witems = 0; /* k_wx items, two int4 each */
   int4* xitems = nullptr; int n_xitems = 0;
+  int4* wred = nullptr; int n_wred = 0;     /* k_wx_reduce items of split lists */
+  int4* xred = nullptr; int n_xred = 0;
+  double* wxpart = nullptr;                 /* chunk sums of split lists, 64 per slot */
   int4* l2t_items = nullptr; int n_l2t = 0;
   size_t chk_cols = 0;
   double* ws = nullptr;
@@ -242,7 +246,8 @@ struct Plan {
   size_t chk_cols = 0;
   size_t ws_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
-  std::vector<int4> witems, xitems;
+  std::vector<int4> witems, xitems, wred, xred; /* two int4 per item */
+  int wx_slots = 0, w_slots = 0, x_slots = 0;     /* partial slots of split W / X lists */
   std::vector<int4> l2t_items;
   std::vector<int4> hgroups;
   std::vector<int> huslots;
@@ -472,28 +477,73 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
         P.l2t_items.push_back(make_int4(i, t->trg_beg[i] + c, std::min(64, t->trg_num[i] - c), 0));
     }
   }
-  /* apply_tile clamps target indices to [tbeg, tbeg + tnum - 1]: the leaves
-   * handed to k_wlist / k_xlist (direct) / k_l2t must own a target */
-  /* W / X items: target chunks of at most WX_CHUNK per block */
-  for (int i : P.wleaves)
-    for (int c = 0; c < t->trg_num[i]; c += WX_CHUNK)
-      P.witems.push_back(make_int4(i, c, std::min(WX_CHUNK, t->trg_num[i] - c), 0));
-  for (int i : P.xnodes) {
-    if (t->child[i] == -1 && t->trg_num[i] < ne) {
-      for (int c = 0; c < t->trg_num[i]; c += WX_CHUNK)
-        P.xitems.push_back(make_int4(i, c, std::min(WX_CHUNK, t->trg_num[i] - c), 1));
-    } else {
-      P.xitems.push_back(make_int4(i, 0, 0, 0));
+  /* W / X items (k_wx): per target node, its list cut at entry boundaries
+   * into chunks of at most WX_CHUNK_SRC sources (an entry larger than that is
+   * a chunk of its own), and its points into blocks of 64; one item per
+   * (block, chunk).  A node with several chunks gets a partial slot per item
+   * and a reduce item per block.  Items run longest chunk first. */
+  auto wx_plan = [&](const std::vector<int>& nodes, const int* ptr, const int* idx, bool xl, std::vector<int4>& items,
+                     std::vector<int4>& red) {
+    std::vector<std::pair<long, std::array<int4, 2>>> work;
+    for (int i : nodes) {
+      const int kind = xl && !(t->child[i] == -1 && t->trg_num[i] < ne) ? 1 : 0;
+      const int npts = kind ? ne : t->trg_num[i];
+      std::vector<int> cut{ptr[i]};
+      std::vector<long> cost;
+      long acc = 0;
+      for (int k = ptr[i]; k < ptr[i + 1]; k++) {
+        const int s_ = idx[k];
+        const long n = xl || (t->child[s_] == -1 && t->src_num[s_] < ne) ? t->src_num[s_] : ne;
+        if (acc > 0 && acc + n > WX_CHUNK_SRC) {
+          cut.push_back(k);
+          cost.push_back(acc);
+          acc = 0;
+        }
+        acc += n;
+      }
+      cut.push_back(ptr[i + 1]);
+      cost.push_back(acc);
+      const int nch = (int)cost.size();
+      for (int c = 0; c < npts; c += 64) {
+        const int4 a = make_int4(i, c, std::min(64, npts - c), kind);
+        const int slot0 = nch > 1 ? P.wx_slots : -1;
+        for (int h = 0; h < nch; h++)
+          work.push_back({cost[h], {a, make_int4(cut[h], cut[h + 1], nch > 1 ? P.wx_slots + h : -1, 0)}});
+        if (nch > 1) {
+          red.push_back(a);
+          red.push_back(make_int4(slot0, nch, 0, 0));
+          P.wx_slots += nch;
+        }
+      }
     }
-  }
-  for (const int4& it : P.witems)
-    if (it.z < 1 || it.y < 0 || it.y + it.z > t->trg_num[it.x]) return fail("W-list item out of range");
-  for (const int4& it : P.xitems)
-    if (it.w && (it.z < 1 || it.y < 0 || it.y + it.z > t->trg_num[it.x])) return fail("X-list item out of range");
-  for (int i : P.wleaves)
-    if (t->trg_num[i] < 1) return fail("W-list leaf without targets");
-  for (int i : P.xnodes)
-    if (t->child[i] == -1 && t->trg_num[i] < ne && t->trg_num[i] < 1) return fail("X-list leaf without targets");
+    std::stable_sort(work.begin(), work.end(), [](const auto& x, const auto& y) { return x.first > y.first; });
+    for (const auto& w : work) {
+      items.push_back(w.second[0]);
+      items.push_back(w.second[1]);
+    }
+  };
+  P.wx_slots = 0;
+  wx_plan(P.wleaves, t->w_ptr, t->w_idx, false, P.witems, P.wred);
+  P.w_slots = P.wx_slots;
+  P.wx_slots = 0;
+  wx_plan(P.xnodes, t->x_ptr, t->x_idx, true, P.xitems, P.xred);
+  P.x_slots = P.wx_slots;
+  /* k_wx clamps point indices to [first, first + count - 1] of its item and
+   * reads trg4 from tbeg[t] (kind 0) or the n_e-point DC surface (kind 1) */
+  auto wx_check = [&](const std::vector<int4>& v, const int* ptr, int slots, bool reduce) {
+    for (size_t m = 0; m < v.size(); m += 2) {
+      const int4 a = v[m], b = v[m + 1];
+      if (a.x < 0 || a.x >= nn || a.z < 1 || a.z > 64 || a.y < 0 || a.y + a.z > (a.w ? ne : t->trg_num[a.x]))
+        return false;
+      if (!reduce && (b.x < ptr[a.x] || b.y > ptr[a.x + 1] || b.x >= b.y || b.z >= slots)) return false;
+      if (reduce && (b.x < 0 || b.y < 2 || b.x + b.y > slots)) return false;
+    }
+    return true;
+  };
+  if (!wx_check(P.witems, t->w_ptr, P.w_slots, false) || !wx_check(P.wred, t->w_ptr, P.w_slots, true))
+    return fail("W-list item out of range");
+  if (!wx_check(P.xitems, t->x_ptr, P.x_slots, false) || !wx_check(P.xred, t->x_ptr, P.x_slots, true))
+    return fail("X-list item out of range");
   for (int i : P.l2t)
     if (t->trg_num[i] < 1) return fail("L2T leaf without targets");
   /* k_l2t reads trg4[it.y .. it.y + it.z - 1] and the leaf's DE densities */
@@ -547,11 +597,8 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     TRY(dev_upload(g, &g->snum, t->src_num, nn));
     TRY(dev_upload(g, &g->sbeg, t->src_beg, nn));
     TRY(dev_upload(g, &g->tbeg, t->trg_beg, nn));
-    TRY(dev_upload(g, &g->tnum, t->trg_num, nn));
     TRY(dev_upload(g, &g->depth, t->depth, nn));
-    TRY(dev_upload(g, &g->wptr, t->w_ptr, nn + 1));
     TRY(dev_upload(g, &g->widx, t->w_idx, t->w_ptr[nn]));
-    TRY(dev_upload(g, &g->xptr, t->x_ptr, nn + 1));
     TRY(dev_upload(g, &g->xidx, t->x_idx, t->x_ptr[nn]));
     TRY(dev_zero(g, &g->uden, (size_t)nn * lde));
     TRY(dev_zero(g, &g->dchk, (size_t)nn * lde));
@@ -665,11 +712,16 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     TRY(dev_zero(g, &g->Y, spec));
     TRY(dev_zero(g, &g->T, (size_t)g->F * 343));
   }
-  g->n_witems = (int)P.witems.size();
-  g->n_xitems = (int)P.xitems.size();
+  g->n_witems = (int)P.witems.size() / 2;
+  g->n_xitems = (int)P.xitems.size() / 2;
+  g->n_wred = (int)P.wred.size() / 2;
+  g->n_xred = (int)P.xred.size() / 2;
   g->n_l2t = (int)P.l2t_items.size();
   TRY(dev_upload(g, &g->witems, P.witems.data(), P.witems.size()));
   TRY(dev_upload(g, &g->xitems, P.xitems.data(), P.xitems.size()));
+  TRY(dev_upload(g, &g->wred, P.wred.data(), P.wred.size()));
+  TRY(dev_upload(g, &g->xred, P.xred.data(), P.xred.size()));
+  TRY(dev_zero(g, &g->wxpart, (size_t)std::max(P.w_slots, P.x_slots) * 64));
   TRY(dev_upload(g, &g->l2t_items, P.l2t_items.data(), P.l2t_items.size()));
   return KIFMM_OK;
 }
@@ -834,16 +886,24 @@ int point_phases_u(kifmm_gpu* g) {
 
 template <bool MASK>
 int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
-  if ((mask & KIFMM_PHASE_W) && g->n_witems > 0)
-    hipLaunchKernelGGL(k_wlist<MASK>, dim3(g->n_witems), dim3(256), 0, g->stream, g->witems, g->wptr, g->widx,
-                       g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_ue, g->ld_surf, g->n_e, g->uden,
-                       g->ld_e, g->src4, g->trg4, g->pot);
+  if ((mask & KIFMM_PHASE_W) && g->n_witems > 0) {
+    hipLaunchKernelGGL((k_wx<MASK, false>), dim3(g->n_witems), dim3(64), 0, g->stream, g->witems, g->widx, g->child,
+                       g->snum, g->sbeg, g->tbeg, g->geo, g->surf_ue, g->surf_dc, g->ld_surf, g->n_e, g->uden,
+                       g->ld_e, g->src4, g->trg4, g->pot, g->dchk, g->wxpart);
+    if (g->n_wred > 0)
+      hipLaunchKernelGGL(k_wx_reduce, dim3(g->n_wred), dim3(64), 0, g->stream, g->wred, g->tbeg, g->ld_e, g->wxpart,
+                         g->pot, g->dchk);
+  }
   LAUNCH_CHECK();
   REC(ew);
-  if ((mask & KIFMM_PHASE_X) && g->n_xitems > 0)
-    hipLaunchKernelGGL(k_xlist<MASK>, dim3(g->n_xitems), dim3(256), 0, g->stream, g->xitems, g->xptr, g->xidx,
-                       g->child, g->snum, g->sbeg, g->tbeg, g->tnum, g->geo, g->surf_dc, g->ld_surf, g->n_e, g->dchk,
-                       g->ld_e, g->src4, g->trg4, g->pot);
+  if ((mask & KIFMM_PHASE_X) && g->n_xitems > 0) {
+    hipLaunchKernelGGL((k_wx<MASK, true>), dim3(g->n_xitems), dim3(64), 0, g->stream, g->xitems, g->xidx, g->child,
+                       g->snum, g->sbeg, g->tbeg, g->geo, g->surf_ue, g->surf_dc, g->ld_surf, g->n_e, g->uden,
+                       g->ld_e, g->src4, g->trg4, g->pot, g->dchk, g->wxpart);
+    if (g->n_xred > 0)
+      hipLaunchKernelGGL(k_wx_reduce, dim3(g->n_xred), dim3(64), 0, g->stream, g->xred, g->tbeg, g->ld_e, g->wxpart,
+                         g->pot, g->dchk);
+  }
   LAUNCH_CHECK();
   return KIFMM_OK;
 }
