@@ -36,10 +36,13 @@ enum {
   KIFMM_GPU_GRAPH = 1, /* capture the whole evaluation in one HIP graph at the first full
                           evaluate, replay it afterwards (phase times then report only
                           total and copy) */
-  KIFMM_GPU_MIXED_INLINE = 2 /* mixed path: test for close pairs inside the U-list loop
-                                instead of listing them once at create time (the list is
-                                skipped by itself when it would exceed 16 pairs per
-                                target on average) */
+  KIFMM_GPU_NO_CLOSE_LIST = 2, /* do not list the U-list pairs closer than 1/8 of the target
+                                  leaf's half-width at create time: the mixed path then tests
+                                  for them inside its loop, the FP64 path forms r^2 from
+                                  coordinate differences (11 instead of 9 instructions per
+                                  pair).  The list is also skipped by itself when it would
+                                  exceed 16 pairs per target on average. */
+  KIFMM_GPU_MIXED_INLINE = KIFMM_GPU_NO_CLOSE_LIST /* the earlier name of the same flag */
 };
 
 /* Launch shapes of a handle, the counterpart of the reference's hand-set
@@ -98,9 +101,9 @@ typedef struct kifmm_work_counts {
   double gemm_flops;   /* 2*m*k per applied operator column         */
   double fft_bytes;    /* HBM bytes of the forward + inverse DFT kernels */
   double hadamard_bytes; /* compulsory HBM bytes of the Hadamard kernel */
-  double close_pairs;  /* mixed path: U-list pairs closer than 1/8 leaf half-width, listed
-                          at create time; -1 when they are tested in the loop instead;
-                          0 on the FP64 path */
+  double close_pairs;  /* U-list pairs closer than 1/8 of the target leaf's half-width,
+                          listed at create time; -1 when not listed (KIFMM_GPU_NO_CLOSE_LIST
+                          or too many) */
   double reserved[4];
 } kifmm_work_counts;
 

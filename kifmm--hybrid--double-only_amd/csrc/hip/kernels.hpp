@@ -178,16 +178,80 @@ __device__ __forceinline__ int walk_tile(RangeWalk& w, const int2* __restrict__ 
   return fill;
 }
 
-template <bool MASK, int G>
+/* FP64 U-list on the Gram form (used when the close pairs are listed, see
+ * k_close_pairs).  Coordinates are centred on the target leaf and scaled by
+ * sc = GRAM_SC / h (a power of two: exact), so a pair is close, r <
+ * CLOSE_FRAC h, exactly when its scaled distance r' is below 2.  Per pair
+ *     r'^2 = (|t|^2 + |s|^2) + (-2t) . s              (1 add, 3 FMA)
+ *     v = clamp01(y (3 - r'^2 y^2)),  y = rsq(|r'^2|)  (rsq, mul, FMA, mul)
+ * then 1 FMA into the sum: 9 VALU instructions against 11 for the
+ * difference form, and v = 2 / r' for every r' >= 2.  For those pairs the
+ * Gram r'^2 is within about eps (2|t| + r')^2 of the exact value, with |t|
+ * <= 16 sqrt(3) here: below 2e-13 relative at r' = 2 and falling with r',
+ * whatever the size of the source leaf.  Each listed close pair then adds
+ * w (2 / r'_exact - v) when its target's sum is written, with v recomputed
+ * by the same instructions (gram_pair) from the same operands, so the
+ * clamped or cancelled main-loop term is replaced exactly (r = 0 with MASK:
+ * 2 / r' taken as 0).  The clamp keeps every main-loop term in [0, 1]:
+ * r'^2 = 0 gives 0 * inf = NaN in the Newton step, which the clamp maps to 0. */
+constexpr double CLOSE_FRAC = 0.125;
+constexpr double GRAM_SC = 2.0 / CLOSE_FRAC;
+constexpr double GRAM_FAR = 1.0e6; /* scaled coordinate of padding sources */
+
+/* (x', y', z', |s'|^2) of point p for leaf centre g and scale sc */
+__device__ __forceinline__ double4 gram_src(double4 p, double4 g, double sc) {
+  const double x = (p.x - g.x) * sc, y = (p.y - g.y) * sc, z = (p.z - g.z) * sc;
+  return make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+}
+/* the main-loop term of target record a = (-2x', -2y', -2z', |t'|^2) and
+ * source record s = gram_src(...) */
+__device__ __forceinline__ double gram_pair(double4 a, double4 s) {
+  const double r2 = __builtin_fma(a.x, s.x, __builtin_fma(a.y, s.y, __builtin_fma(a.z, s.z, a.w + s.w)));
+  const double y = __builtin_amdgcn_rsq(__builtin_fabs(r2));
+  const double v = y * __builtin_fma(-r2, y * y, 3.0);
+  return __builtin_fmin(__builtin_fmax(v, 0.0), 1.0);
+}
+
+/* acc[t] += w * gram_pair(a[t], s) over LDS sources [0, ns) (ns a multiple of 64) */
+__device__ __forceinline__ void eval8_gram(const double4* __restrict__ src, const double* __restrict__ sw, int ns,
+                                           const double4 (&a)[8], double acc[8], int lane) {
+  for (int s = lane; s < ns; s += 64) {
+    const double4 p = src[s];
+    const double w = sw[s];
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = __builtin_fma(w, gram_pair(a[t], p), acc[t]);
+  }
+}
+
+/* Scratch of the Gram form: the scaled source weights of a tile, the close-
+ * pair sums of an item's targets (LDS), and the per-target records
+ * (-2t', |t'|^2) with the close-pair CSR list (device memory). */
+struct GramArgs {
+  double* sw;
+  double* sc_sum;
+  const double4* __restrict__ geo;
+  const double4* __restrict__ trg_gram;
+  const int* __restrict__ cptr;
+  const int* __restrict__ cidx;
+};
+
+template <bool MASK, int G, bool GRAM>
 __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, const int* __restrict__ rng_ptr,
                                          const int2* __restrict__ rng, const double4* __restrict__ src4,
-                                         const double4* __restrict__ trg4, double* __restrict__ pot) {
+                                         const double4* __restrict__ trg4, double* __restrict__ pot,
+                                         const GramArgs& ga) {
   const int tid = threadIdx.x, lane = tid & 63;
   /* groups g = wave' + 4k with wave' the wave index rotated by the item:
    * when the group count is not a multiple of 4 the short wave moves from
    * SIMD to SIMD instead of always idling the same one */
   const int wave = (__builtin_amdgcn_readfirstlane(tid >> 6) + rot) & 3;
   const int ngroups = (it.z + 7) >> 3;
+  double4 g = make_double4(0.0, 0.0, 0.0, 1.0);
+  double sc = 1.0;
+  if constexpr (GRAM) {
+    g = ga.geo[it.w];
+    sc = GRAM_SC / g.w;
+  }
   double acc[G][8];
 #pragma unroll
   for (int k = 0; k < G; k++)
@@ -205,7 +269,15 @@ __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, 
     while (fill < P2P_TILE && ri < rlast) {
       const int2 r = rng[ri];
       const int take = min(r.y - ro, P2P_TILE - fill);
-      for (int k = tid; k < take; k += 256) tile[fill + k] = src4[r.x + ro + k];
+      for (int k = tid; k < take; k += 256) {
+        const double4 q = src4[r.x + ro + k];
+        if constexpr (GRAM) {
+          tile[fill + k] = gram_src(q, g, sc);
+          ga.sw[fill + k] = q.w;
+        } else {
+          tile[fill + k] = q;
+        }
+      }
       fill += take;
       ro += take;
       if (ro == r.y) {
@@ -214,20 +286,74 @@ __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, 
       }
     }
     const int ns = (fill + 63) & ~63;
-    for (int k = fill + tid; k < ns; k += 256) tile[k] = make_double4(FAR, FAR, FAR, 0.0);
+    for (int k = fill + tid; k < ns; k += 256) {
+      if constexpr (GRAM) {
+        tile[k] = make_double4(GRAM_FAR, GRAM_FAR, GRAM_FAR, 3.0 * GRAM_FAR * GRAM_FAR);
+        ga.sw[k] = 0.0;
+      } else {
+        tile[k] = make_double4(FAR, FAR, FAR, 0.0);
+      }
+    }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < G; k++) {
       const int grp = wave + 4 * k;
       if (grp < ngroups) {
         /* slots past the count re-read the item's last target (in range) */
-        double tx[8], ty[8], tz[8];
+        if constexpr (GRAM) {
+          double4 a[8];
 #pragma unroll
-        for (int t = 0; t < 8; t++) {
-          const double4 q = trg4[it.y + min(8 * grp + t, it.z - 1)];
-          tx[t] = q.x; ty[t] = q.y; tz[t] = q.z;
+          for (int t = 0; t < 8; t++) a[t] = ga.trg_gram[it.y + min(8 * grp + t, it.z - 1)];
+          eval8_gram(tile, ga.sw, ns, a, acc[k], lane);
+        } else {
+          double tx[8], ty[8], tz[8];
+#pragma unroll
+          for (int t = 0; t < 8; t++) {
+            const double4 q = trg4[it.y + min(8 * grp + t, it.z - 1)];
+            tx[t] = q.x; ty[t] = q.y; tz[t] = q.z;
+          }
+          eval8_lds<MASK>(tile, ns, tx, ty, tz, acc[k], lane);
         }
-        eval8_lds<MASK>(tile, ns, tx, ty, tz, acc[k], lane);
+      }
+    }
+  }
+  if constexpr (GRAM) {
+    /* close-pair terms: a group's pairs are one contiguous run of cidx
+     * (targets in order), spread over the 64 lanes and summed per target by
+     * a segmented scan over the lanes (fixed order: deterministic) */
+    const double isc = g.w / GRAM_SC; /* 1 / sc, exact */
+#pragma unroll
+    for (int k = 0; k < G; k++) {
+      const int grp = wave + 4 * k;
+      if (grp < ngroups) {
+        const int t0 = 8 * grp, nt = min(8, it.z - t0), tg0 = it.y + t0;
+        if (lane < 8) ga.sc_sum[t0 + lane] = 0.0;
+        const int kbeg = ga.cptr[tg0], kend = ga.cptr[tg0 + nt];
+        const int ej = ga.cptr[tg0 + min(lane & 7, nt - 1) + 1]; /* end of target lane&7's run */
+        for (int kb = kbeg; kb < kend; kb += 64) {
+          const int kk = kb + lane;
+          const bool live = kk < kend;
+          int tl = 0; /* the target of pair kk: the number of runs that end at or before kk */
+#pragma unroll
+          for (int j = 0; j < 8; j++) tl += __builtin_amdgcn_readlane(ej, j) <= kk;
+          tl = live ? tl : 8;
+          double v = 0.0;
+          if (live) {
+            const double4 q = src4[ga.cidx[kk]];
+            const double4 tp = trg4[tg0 + tl];
+            const double dx = tp.x - q.x, dy = tp.y - q.y, dz = tp.z - q.z;
+            const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+            v = q.w * (inv_r2<MASK>(r2) * isc - gram_pair(ga.trg_gram[tg0 + tl], gram_src(q, g, sc)));
+          }
+#pragma unroll
+          for (int off = 1; off < 64; off <<= 1) {
+            const double u = __shfl_up(v, off, 64);
+            const int ut = __shfl_up(tl, off, 64);
+            if (lane >= off && ut == tl) v += u;
+          }
+          const int nx = __shfl_down(tl, 1, 64);
+          if (live && (lane == 63 || nx != tl)) ga.sc_sum[t0 + tl] += v; /* one writer per target and chunk */
+        }
       }
     }
   }
@@ -237,23 +363,32 @@ __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, 
     if (grp < ngroups) {
       const double sum = wave_reduce8(acc[k], lane);
       const int t = 8 * grp + (lane >> 3);
-      if ((lane & 7) == 0 && t < it.z) pot[it.y + t] = HALF_OOFP_R * sum;
+      if ((lane & 7) == 0 && t < it.z) {
+        if constexpr (GRAM) pot[it.y + t] = (HALF_OOFP_R * sc) * (sum + ga.sc_sum[t]);
+        else pot[it.y + t] = HALF_OOFP_R * sum;
+      }
     }
   }
 }
 
 /* One block per item of at most 32 G targets. */
-template <bool MASK, int G>
+template <bool MASK, int G, bool GRAM>
 __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ items, int n_items, int per_xcd,
                                                    const int* __restrict__ rng_ptr, const int2* __restrict__ rng,
                                                    const double4* __restrict__ src4,
-                                                   const double4* __restrict__ trg4, double* __restrict__ pot) {
+                                                   const double4* __restrict__ trg4, double* __restrict__ pot,
+                                                   const double4* __restrict__ geo,
+                                                   const double4* __restrict__ trg_gram,
+                                                   const int* __restrict__ cptr, const int* __restrict__ cidx) {
   __shared__ double4 tile[P2P_TILE];
+  __shared__ double sw[GRAM ? P2P_TILE : 1];
+  __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
   /* workgroups go round-robin to the 8 XCDs: give each XCD a contiguous run
    * of items (neighbouring leaves) so their shared sources stay in its L2 */
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
-  p2p_item<MASK, G>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot);
+  const GramArgs ga{sw, sc_sum, geo, trg_gram, cptr, cidx};
+  p2p_item<MASK, G, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
 }
 
 /* Persistent form over the same items (<= P2P_MAXT targets): a fixed grid
@@ -261,25 +396,46 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
  * contiguous run one at a time from counter x (zeroed before the launch),
  * so uneven items even out inside each XCD.  Every block leaves the loop
  * at its first draw past the run's end; the counters only grow. */
-template <bool MASK>
+template <bool MASK, bool GRAM>
 __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restrict__ items, int n_items, int per_xcd,
                                                            const int* __restrict__ rng_ptr,
                                                            const int2* __restrict__ rng,
                                                            const double4* __restrict__ src4,
                                                            const double4* __restrict__ trg4, double* __restrict__ pot,
-                                                           int* __restrict__ counters) {
+                                                           int* __restrict__ counters,
+                                                           const double4* __restrict__ geo,
+                                                           const double4* __restrict__ trg_gram,
+                                                           const int* __restrict__ cptr, const int* __restrict__ cidx) {
   __shared__ double4 tile[P2P_TILE];
+  __shared__ double sw[GRAM ? P2P_TILE : 1];
+  __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
   __shared__ int draw;
   const int x = (int)(blockIdx.x & 7);
   const int lo = x * per_xcd, n = min(n_items - lo, per_xcd);
+  const GramArgs ga{sw, sc_sum, geo, trg_gram, cptr, cidx};
   for (;;) {
     if (threadIdx.x == 0) draw = atomicAdd(counters + x, 1);
     __syncthreads();
     const int k = draw;
     __syncthreads(); /* every thread has read draw before the next draw overwrites it */
     if (k >= n) break;
-    p2p_item<MASK, P2P_G>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot);
+    p2p_item<MASK, P2P_G, GRAM>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot, ga);
   }
+}
+
+/* The Gram form's per-target records (-2t', |t'|^2), t' = (t - c) sc for the
+ * target's leaf (centre c, sc = GRAM_SC / h): fixed for the handle's
+ * lifetime, written once at create time.  One block per U-list item. */
+static __global__ __launch_bounds__(128) void k_trg_gram(const int4* __restrict__ items,
+                                                        const double4* __restrict__ geo,
+                                                        const double4* __restrict__ trg4,
+                                                        double4* __restrict__ trg_gram) {
+  const int4 it = items[blockIdx.x];
+  const int k = threadIdx.x;
+  if (k >= it.z) return;
+  const double4 g = geo[it.w];
+  const double4 s = gram_src(trg4[it.y + k], g, GRAM_SC / g.w);
+  trg_gram[it.y + k] = make_double4(-2.0 * s.x, -2.0 * s.y, -2.0 * s.z, s.w);
 }
 
 static __global__ void k_fill_int(int* __restrict__ p, int n, int v) {
@@ -476,7 +632,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
  * MASK), which replaces the clamped w by the FP64 term.  Pairs near r' = 1
  * whose FP32 and FP64 sides disagree are off by the FP32 error of a pair
  * at distance tau h (~1e-4 relative to that pair, as for any FP32 pair). */
-constexpr double MIX_CLOSE = 0.125;
+constexpr double MIX_CLOSE = CLOSE_FRAC;
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 /* FP32(v) = h + m + l exactly: each remainder is exact in FP32 and the

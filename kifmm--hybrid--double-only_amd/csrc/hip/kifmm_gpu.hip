@@ -82,7 +82,7 @@ struct kifmm_gpu {
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
   int* rng_ptr = nullptr; int2* rng = nullptr;
   int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
-  int persist_grid = 0;
+  int persist_grid[2] = {0, 0}; /* blocks of the persistent U-list form: difference, Gram */
   /* launch shapes (kifmm_gpu_set_tuning / kifmm_gpu_autotune) */
   int mix_lpb = 1; /* leaves per block of the mixed S2M / L2T */
   int p2p_form = KIFMM_P2P_STATIC;
@@ -91,6 +91,7 @@ struct kifmm_gpu {
   /* mixed path: per target, the U-list sources closer than MIX_CLOSE h */
   int* cptr = nullptr; int* cidx = nullptr;
   int close_listed = 0;
+  double4* trg_gram = nullptr; /* FP64 Gram-form U-list: (-2t', |t'|^2) per target, when close_listed */
   long long n_close = 0;
   int4* s2m_items = nullptr; int n_s2m = 0;
   GemmList s2m_gemm;
@@ -673,8 +674,10 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   {
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false>, 256, 0));
-    g->persist_grid = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, false>, 256, 0));
+    g->persist_grid[0] = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, true>, 256, 0));
+    g->persist_grid[1] = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
   }
   TRY(dev_upload(g, &g->rng_ptr, P.rng_ptr.data(), P.rng_ptr.size()));
   TRY(dev_upload(g, &g->rng, P.rng.data(), P.rng.size()));
@@ -740,9 +743,10 @@ int mixed_setup(kifmm_gpu* g) {
   return KIFMM_OK;
 }
 
-/* Close pairs of the mixed U-list (k_close_pairs): counts, offsets on the
- * host, then the indices.  An input whose list would exceed 16 pairs per
- * target on average keeps the in-loop test (k_p2p_ulist_mixed). */
+/* Close pairs of the U-list (k_close_pairs): counts, offsets on the host,
+ * then the indices.  An input whose list would exceed 16 pairs per target
+ * on average is not listed: the mixed path then keeps the in-loop test
+ * (k_p2p_ulist_mixed), the FP64 path the difference form. */
 int close_list(kifmm_gpu* g) {
   int* cnt = nullptr;
   TRY(dev_zero(g, &cnt, (size_t)std::max(g->n_trg, 1)));
@@ -772,6 +776,18 @@ int close_list(kifmm_gpu* g) {
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->n_close = tot;
   g->close_listed = 1;
+  return KIFMM_OK;
+}
+
+/* FP64 path with listed close pairs: the U-list runs on the Gram form
+ * (k_p2p_ulist<., ., true>), whose per-target records are written here. */
+int gram_setup(kifmm_gpu* g) {
+  TRY(dev_zero(g, &g->trg_gram, (size_t)std::max(g->n_trg, 1)));
+  if (g->n_p2p > 0)
+    hipLaunchKernelGGL(k_trg_gram, dim3(g->n_p2p), dim3(128), 0, g->stream, g->p2p_items, g->geo, g->trg4,
+                       g->trg_gram);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream));
   return KIFMM_OK;
 }
 
@@ -857,6 +873,27 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   return vlist_phase<NP, double2>(g, g->X, g->Y, g->T, e1, e2, e3);
 }
 
+template <bool MASK, bool GRAM>
+int p2p_fp64(kifmm_gpu* g, int per_xcd) {
+  if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, g->stream, g->p2p_counters, 8, 0);
+    hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, GRAM>), dim3(g->persist_grid[GRAM]), dim3(256), 0, g->stream,
+                       g->p2p_items, g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters,
+                       g->geo, g->trg_gram, g->cptr, g->cidx);
+  } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
+    const int per64 = (g->n_p2p64 + 7) / 8;
+    hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G / 2, GRAM>), dim3(8 * per64), dim3(256), 0, g->stream,
+                       g->p2p_items64, g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo,
+                       g->trg_gram, g->cptr, g->cidx);
+  } else {
+    hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G, GRAM>), dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
+                       g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo, g->trg_gram, g->cptr,
+                       g->cidx);
+  }
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
 template <bool MASK>
 int point_phases_u(kifmm_gpu* g) {
   if (g->n_p2p > 0) {
@@ -868,17 +905,10 @@ int point_phases_u(kifmm_gpu* g) {
     else if (g->precision == KIFMM_PREC_MIXED)
       hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
                          g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
-    else if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
-      hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, g->stream, g->p2p_counters, 8, 0);
-      hipLaunchKernelGGL(k_p2p_ulist_persist<MASK>, dim3(g->persist_grid), dim3(256), 0, g->stream, g->p2p_items,
-                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters);
-    } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
-      const int per64 = (g->n_p2p64 + 7) / 8;
-      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G / 2>), dim3(8 * per64), dim3(256), 0, g->stream, g->p2p_items64,
-                         g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
-    } else
-      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G>), dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
-                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot);
+    else if (g->trg_gram)
+      return p2p_fp64<MASK, true>(g, per_xcd);
+    else
+      return p2p_fp64<MASK, false>(g, per_xcd);
   }
   LAUNCH_CHECK();
   return KIFMM_OK;
@@ -1119,8 +1149,8 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
   try {
     r = build(g, tree, ops);
     if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED) r = mixed_setup(g);
-    if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED && !(opts && (opts->flags & KIFMM_GPU_MIXED_INLINE)))
-      r = close_list(g);
+    if (r == KIFMM_OK && !(opts && (opts->flags & KIFMM_GPU_NO_CLOSE_LIST))) r = close_list(g);
+    if (r == KIFMM_OK && g->precision == KIFMM_PREC_FP64 && g->close_listed) r = gram_setup(g);
   } catch (const std::bad_alloc&) {
     set_error("host allocation failed in kifmm_gpu_create");
     r = KIFMM_ENOMEM;
@@ -1337,7 +1367,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
 int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out) {
   if (!g || !out) return KIFMM_EINVAL;
   *out = g->counts;
-  out->close_pairs = g->precision != KIFMM_PREC_MIXED ? 0.0 : g->close_listed ? (double)g->n_close : -1.0;
+  out->close_pairs = g->close_listed ? (double)g->n_close : -1.0;
   return KIFMM_OK;
 }
 

@@ -24,7 +24,8 @@ KIFMM_TRGNODE = 2
 ERRORS = {-1: "EINVAL", -2: "ENOMEM", -3: "EDEVICE", -4: "EUNSUP"}
 PREC_FP64, PREC_MIXED = 0, 1
 GPU_GRAPH = 1
-GPU_MIXED_INLINE = 2
+GPU_NO_CLOSE_LIST = 2
+GPU_MIXED_INLINE = GPU_NO_CLOSE_LIST  # earlier name of the same flag
 PHASE_UP, PHASE_U, PHASE_V, PHASE_W, PHASE_X, PHASE_DOWN, PHASE_ALL = 1, 2, 4, 8, 16, 32, 63
 
 
@@ -333,12 +334,13 @@ class GpuEvaluator:
 
     def __init__(self, problem: Problem, device=0, precision=PREC_FP64, operators: Operators | None = None,
                  graph=False, inline_close=False):
-        """inline_close: on the mixed path, test for close U-list pairs inside
-        the kernel loop instead of listing them at create time."""
+        """inline_close: do not list the close U-list pairs at create time;
+        the mixed path then tests for them inside its loop and the FP64 path
+        forms r^2 from coordinate differences instead of the Gram form."""
         self.problem = problem
         self._view = problem.view()
         self._ops = operators if operators is not None else problem.operators()
-        opts = GpuOptions(device, precision, (GPU_GRAPH if graph else 0) | (GPU_MIXED_INLINE if inline_close else 0))
+        opts = GpuOptions(device, precision, (GPU_GRAPH if graph else 0) | (GPU_NO_CLOSE_LIST if inline_close else 0))
         h = C.c_void_p()
         _check(lib.kifmm_gpu_create(C.byref(self._view), C.byref(self._ops), C.byref(opts), C.byref(h)))
         self._h = h
