@@ -844,19 +844,23 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
   }
 }
 
-/* The close pairs of the mixed U-list (geometry only, found once per
- * plan): for each target of an item, the sources of its U-list ranges at
- * FP64 scaled distance r' < 1 (r < MIX_CLOSE h), in range order.
- * FILL = false writes the count per target; FILL = true writes the source
- * indices from the exclusive offsets cptr. */
+/* The close pairs of the U-list (geometry only, found once per plan): for
+ * each target of an item, the sources of its U-list ranges at FP64 scaled
+ * distance r' < 1 (r < MIX_CLOSE h), in range order.  FILL = false writes
+ * the count per target; FILL = true writes the source indices from the
+ * exclusive offsets cptr.  Items in the U-list kernels' XCD-contiguous
+ * order (8 per_xcd blocks), so neighbouring leaves share an L2. */
 template <bool FILL>
-__global__ __launch_bounds__(128) void k_close_pairs(const int4* __restrict__ items, const int* __restrict__ rng_ptr,
+__global__ __launch_bounds__(128) void k_close_pairs(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                     const int* __restrict__ rng_ptr,
                                                      const int2* __restrict__ rng, const double4* __restrict__ geo,
                                                      const double4* __restrict__ src4,
                                                      const double4* __restrict__ trg4, int* __restrict__ cnt,
                                                      const int* __restrict__ cptr, int* __restrict__ cidx) {
   __shared__ double4 tile[P2P_TILE];
-  const int4 it = items[blockIdx.x];
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const int4 it = items[item];
   const double4 g = geo[it.w];
   const double sc = (1.0 / MIX_CLOSE) / g.w;
   const int k = threadIdx.x;

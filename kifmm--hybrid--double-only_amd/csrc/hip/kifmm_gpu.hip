@@ -751,8 +751,9 @@ int close_list(kifmm_gpu* g) {
   int* cnt = nullptr;
   TRY(dev_zero(g, &cnt, (size_t)std::max(g->n_trg, 1)));
   if (g->n_p2p > 0)
-    hipLaunchKernelGGL(k_close_pairs<false>, dim3(g->n_p2p), dim3(128), 0, g->stream, g->p2p_items, g->rng_ptr,
-                       g->rng, g->geo, g->src4, g->trg4, cnt, nullptr, nullptr);
+    hipLaunchKernelGGL(k_close_pairs<false>, dim3(8 * ((g->n_p2p + 7) / 8)), dim3(128), 0, g->stream, g->p2p_items,
+                       g->n_p2p, (g->n_p2p + 7) / 8, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, cnt, nullptr,
+                       nullptr);
   HIP_TRY(hipGetLastError());
   std::vector<int> h((size_t)std::max(g->n_trg, 1));
   HIP_TRY(hipMemcpyAsync(h.data(), cnt, sizeof(int) * h.size(), hipMemcpyDeviceToHost, g->stream));
@@ -770,8 +771,9 @@ int close_list(kifmm_gpu* g) {
   TRY(dev_upload(g, &g->cptr, ptr.data(), ptr.size()));
   TRY(dev_zero(g, &g->cidx, (size_t)std::max<long long>(tot, 1)));
   if (g->n_p2p > 0)
-    hipLaunchKernelGGL(k_close_pairs<true>, dim3(g->n_p2p), dim3(128), 0, g->stream, g->p2p_items, g->rng_ptr,
-                       g->rng, g->geo, g->src4, g->trg4, nullptr, g->cptr, g->cidx);
+    hipLaunchKernelGGL(k_close_pairs<true>, dim3(8 * ((g->n_p2p + 7) / 8)), dim3(128), 0, g->stream, g->p2p_items,
+                       g->n_p2p, (g->n_p2p + 7) / 8, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, nullptr, g->cptr,
+                       g->cidx);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->n_close = tot;
