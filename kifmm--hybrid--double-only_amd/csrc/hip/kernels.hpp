@@ -391,6 +391,27 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
   p2p_item<MASK, G, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
 }
 
+/* The same with items of at most 64 targets (G = 2 groups per wave): the
+ * compiler is held to 6 waves per SIMD, which these 16 accumulators per
+ * lane allow without spills (71 VGPRs on the Gram form, against 104 when it
+ * is left free): 5.89 -> 5.64 ms at 4M points.  (For the 128-target items
+ * more waves did not pay: 8 waves x 2 groups per block ran 5.75 ms against
+ * 5.61 for 4 x 4.) */
+template <bool MASK, bool GRAM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) void k_p2p_ulist64(
+    const int4* __restrict__ items, int n_items, int per_xcd, const int* __restrict__ rng_ptr,
+    const int2* __restrict__ rng, const double4* __restrict__ src4, const double4* __restrict__ trg4,
+    double* __restrict__ pot, const double4* __restrict__ geo, const double4* __restrict__ trg_gram,
+    const int* __restrict__ cptr, const int* __restrict__ cidx) {
+  __shared__ double4 tile[P2P_TILE];
+  __shared__ double sw[GRAM ? P2P_TILE : 1];
+  __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const GramArgs ga{sw, sc_sum, geo, trg_gram, cptr, cidx};
+  p2p_item<MASK, P2P_G / 2, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
+}
+
 /* Persistent form over the same items (<= P2P_MAXT targets): a fixed grid
  * of a few blocks per CU; the blocks of XCD x draw the items of x's
  * contiguous run one at a time from counter x (zeroed before the launch),

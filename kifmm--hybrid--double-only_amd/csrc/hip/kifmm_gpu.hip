@@ -884,7 +884,7 @@ int p2p_fp64(kifmm_gpu* g, int per_xcd) {
                        g->geo, g->trg_gram, g->cptr, g->cidx);
   } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
     const int per64 = (g->n_p2p64 + 7) / 8;
-    hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G / 2, GRAM>), dim3(8 * per64), dim3(256), 0, g->stream,
+    hipLaunchKernelGGL((k_p2p_ulist64<MASK, GRAM>), dim3(8 * per64), dim3(256), 0, g->stream,
                        g->p2p_items64, g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo,
                        g->trg_gram, g->cptr, g->cidx);
   } else {

@@ -120,7 +120,8 @@ def test_dense_close_pairs_fall_back_to_the_inline_test():
     """Groups of 32 points within 1e-6 of each other make every target's
     close list >= 31 long, over the 16-per-target cap: the evaluator keeps
     the in-loop test by itself (same result as asking for it) and stays
-    within 1e-5 of the FP64 path."""
+    within 1e-5 of the FP64 path, which falls back to its difference form
+    (bit for bit what asking for it gives)."""
     rng = np.random.default_rng(12)
     cen = rng.uniform(-0.9, 0.9, (2500, 3))
     pts = (np.repeat(cen, 32, axis=0) + rng.uniform(-1e-6, 1e-6, (80000, 3))).T
@@ -130,6 +131,9 @@ def test_dense_close_pairs_fall_back_to_the_inline_test():
     assert auto.work_counts()["close_pairs"] == -1
     a, _ = auto.evaluate()
     b, _ = K.GpuEvaluator(prob, precision=K.PREC_MIXED, inline_close=True).evaluate()
-    d, _ = K.GpuEvaluator(prob).evaluate()
-    assert np.array_equal(a, b)
+    fp64 = K.GpuEvaluator(prob)
+    assert fp64.work_counts()["close_pairs"] == -1
+    d, _ = fp64.evaluate()
+    e, _ = K.GpuEvaluator(prob, inline_close=True).evaluate()
+    assert np.array_equal(a, b) and np.array_equal(d, e)
     assert rel(a, d) < 1e-5

@@ -30,6 +30,8 @@ CASES = [
     (20000, "gaussmix", 32, 4, 3, 0),
     (40000, "ellipsoid", 24, 6, 9, 1),
     (60000, "uniform", 300, 4, 4, 0),         # leaves of 130-300 targets: several U-list items per leaf
+    (60000, "gaussmix", 256, 4, 3, 0),        # W and X lists over 1024 sources: chunked, k_wx_reduce
+    (40000, "gaussmix", 512, 3, 1, 1),        # chunked X lists with coincident targets
 ]
 
 
