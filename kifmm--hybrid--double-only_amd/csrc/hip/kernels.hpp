@@ -1648,7 +1648,9 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
     __syncthreads();
     if (fi + 1 < FB) fetch(f + 1);
     if (!active) continue;
-    V accr = {0, 0, 0, 0}, acci = {0, 0, 0, 0};
+    /* four independent accumulator chains: no MFMA waits on the one just
+     * issued */
+    V accr0 = {0, 0, 0, 0}, acci0 = {0, 0, 0, 0}, accr1 = {0, 0, 0, 0}, acci1 = {0, 0, 0, 0};
 #pragma unroll 2
     for (int o = 0; o < 27; o++) {
       if (o == 13) continue; /* the parent itself: its children are all adjacent */
@@ -1656,12 +1658,12 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
       const ST t0 = Ts[b0 + os], t1 = Ts[b1 + os];
       const ST* xs = Xs[sloc[o][pj]];
       const ST x0 = xs[kk], x1 = xs[kk + 4];
-      accr = mfma4(rrow ? t0.x : t0.y, x0.x, accr);
-      accr = mfma4(rrow ? t1.x : t1.y, x1.x, accr);
-      acci = mfma4(rrow ? -t0.y : t0.x, x0.y, acci);
-      acci = mfma4(rrow ? -t1.y : t1.x, x1.y, acci);
+      accr0 = mfma4(rrow ? t0.x : t0.y, x0.x, accr0);
+      acci0 = mfma4(rrow ? -t0.y : t0.x, x0.y, acci0);
+      accr1 = mfma4(rrow ? t1.x : t1.y, x1.x, accr1);
+      acci1 = mfma4(rrow ? -t1.y : t1.x, x1.y, acci1);
     }
-    const V acc = accr + acci;
+    const V acc = (accr0 + accr1) + (acci0 + acci1);
     ST* yb = Y + ((size_t)ys * F + f) * 8;
     if constexpr (sizeof(R) == 8) {
       /* D[i][j] in lane j + 16(i % 4), element i / 4: rows kk, kk+4 (Re)
