@@ -59,6 +59,9 @@ def parse():
     ap.add_argument("--no-gpu-tree", action="store_true", help="skip timing the GPU tree build")
     ap.add_argument("--no-large", action="store_true", help="skip the 16M-point mixed-vs-FP64 case")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--overlap", action="store_true",
+                    help="time every path with KIFMM_GPU_OVERLAP (the U-list on a second stream alongside up "
+                         "and V); by default only the extra 'overlap' figure uses it")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB five times (counter calibration) and exit")
     return ap.parse_args()
@@ -110,10 +113,13 @@ def cpu_baseline(args, n_threads):
                       f"W {t['wlist']:.2f} X {t['xlist']:.2f} down {t['down']:.2f}"}
 
 
-def timed(K, prob, precision, steps, warmup, graph=False):
+OVERLAP = False  # KIFMM_GPU_OVERLAP for every timed evaluator (bench.py --overlap)
+
+
+def timed(K, prob, precision, steps, warmup, graph=False, overlap=None):
     """Mean wall ms per evaluation (inputs resident, result left on the
     device) and mean HIP-event phase times."""
-    ev = K.GpuEvaluator(prob, precision=precision, graph=graph)
+    ev = K.GpuEvaluator(prob, precision=precision, graph=graph, overlap=OVERLAP if overlap is None else overlap)
     for _ in range(warmup):
         ev.evaluate(copy=False)
     phases = []
@@ -163,7 +169,7 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     tuner = K.GpuEvaluator(mprob, precision=prec)
     shapes, _ = tuner.autotune(reps=3)
     del tuner
-    ev = K.GpuEvaluator(mprob, precision=prec, graph=True)
+    ev = K.GpuEvaluator(mprob, precision=prec, graph=True, overlap=OVERLAP)
     ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"], shapes["hadamard_form"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
@@ -200,7 +206,7 @@ def tuned_nonuniform(K, steps, warmup, nv):
     best, cand = tuner.autotune(reps=3)
     t_tune = time.perf_counter() - t0
     del tuner
-    ev = K.GpuEvaluator(prob, graph=True)
+    ev = K.GpuEvaluator(prob, graph=True, overlap=OVERLAP)
     ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
@@ -252,7 +258,9 @@ def large_mixed(K, steps, warmup, nv):
 
 
 def main():
+    global OVERLAP
     args = parse()
+    OVERLAP = args.overlap
     if args.gpus != 1:
         print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
               f"(the multi-GPU sharding is a design in DESIGN.md, not a code path).", file=sys.stderr)
@@ -296,11 +304,19 @@ def main():
                 "kernel": "k_p2p_ulist", "achieved": achieved, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / FP64_PEAK_TFLOPS, "traffic": read_traffic(),
                 "algorithmic": f"{P2P_FLOPS_PER_PAIR:g} flop/pair x {wc['u_pairs']:.4g} U-list pairs per launch",
-                "kernel_ms": p2p_ms}
+                "kernel_ms": p2p_ms,
+                "kernel_shares_gpu": OVERLAP}
+    del ev
+    # the same input with the other stream layout: KIFMM_GPU_OVERLAP runs the
+    # U-list alongside up and V (faster overall; the U-list kernel then
+    # shares the chip and its own span grows)
+    ev_o, ms_o, avg_o = timed(K, prob, prec, args.steps, args.warmup, overlap=not OVERLAP)
+    del ev_o
+    other = {"overlap": not OVERLAP, "ms_per_step": ms_o, "mpoints_s": args.n / (ms_o * 1e-3) / 1e6,
+             "p2p_kernel_ms": avg_o["p2p_kernel"]}
 
     mixed = None
     if not args.no_mixed_compare:
-        del ev
         mixed = mixed_vs_fp64(K, max(5, args.steps // 2), args.warmup, args.nv)
 
     large = None
@@ -344,6 +360,7 @@ def main():
         "tuned_nonuniform": tuned,
         "large_mixed": large,
         "roofline": roofline,
+        "stream_layout": {"overlap": OVERLAP, "other_layout": other},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -42,7 +42,11 @@ enum {
                                   coordinate differences (11 instead of 9 instructions per
                                   pair).  The list is also skipped by itself when it would
                                   exceed 16 pairs per target on average. */
-  KIFMM_GPU_MIXED_INLINE = KIFMM_GPU_NO_CLOSE_LIST /* the earlier name of the same flag */
+  KIFMM_GPU_MIXED_INLINE = KIFMM_GPU_NO_CLOSE_LIST, /* the earlier name of the same flag */
+  KIFMM_GPU_OVERLAP = 4 /* run the U-list on a second stream of the handle (default priority)
+                           alongside the up pass and the V-list, joined before W, X and down;
+                           the potentials are bit-identical to the one-stream order.  Phase
+                           times then overlap: ulist is the U-list's own span. */
 };
 
 /* Launch shapes of a handle, the counterpart of the reference's hand-set

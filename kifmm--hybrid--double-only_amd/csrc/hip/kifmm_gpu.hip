@@ -60,6 +60,11 @@ struct kifmm_gpu {
   int dev = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev[16] = {};
+  /* KIFMM_GPU_OVERLAP: the U-list on a second stream (default priority)
+   * alongside up and V; fork / join events and the U-list's own timing */
+  int overlap = 0;
+  hipStream_t side = nullptr;
+  hipEvent_t fork_ev = nullptr, join_ev = nullptr, su0 = nullptr, su1 = nullptr;
   int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
   int N = 0, H = 0, F = 0;
   int n_nodes = 0, n_src = 0, n_trg = 0, coincident = 0, max_depth = 0, precision = 0;
@@ -950,6 +955,24 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
   TRY(fill(g, g->dchk, (size_t)g->n_nodes * lde));
   TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
+  /* KIFMM_GPU_OVERLAP: the U-list forks onto the side stream here and runs
+   * alongside up and V.  It reads only points, lists and create-time data
+   * and writes pot with '=' (the pot fill above is ordered before the
+   * fork); nothing on this stream touches pot until the join, placed before
+   * W, X and down, which add into pot. */
+  const bool split = g->overlap && (mask & KIFMM_PHASE_U);
+  if (split) {
+    HIP_TRY(hipEventRecord(g->fork_ev, g->stream));
+    HIP_TRY(hipStreamWaitEvent(g->side, g->fork_ev, 0));
+    if (!g->capturing) HIP_TRY(hipEventRecord(g->su0, g->side));
+    hipStream_t main_stream = g->stream;
+    g->stream = g->side;
+    const int r = g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g);
+    g->stream = main_stream;
+    TRY(r);
+    if (!g->capturing) HIP_TRY(hipEventRecord(g->su1, g->side));
+    HIP_TRY(hipEventRecord(g->join_ev, g->side));
+  }
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
@@ -969,7 +992,7 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
   }
   REC(ev[1]);
   /* U-list */
-  if (mask & KIFMM_PHASE_U) TRY(g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g));
+  if ((mask & KIFMM_PHASE_U) && !split) TRY(g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g));
   REC(ev[2]);
   /* V-list */
   if (mask & KIFMM_PHASE_V) switch (g->np) {
@@ -983,6 +1006,7 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
     REC(ev[4]);
     REC(ev[5]);
   }
+  if (split) HIP_TRY(hipStreamWaitEvent(g->stream, g->join_ev, 0));
   REC(ev[6]);
   /* W and X */
   if (mask & (KIFMM_PHASE_W | KIFMM_PHASE_X))
@@ -1072,6 +1096,11 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
     times->wlist = ms[7];
     times->xlist = ms[8];
     times->down = ms[9];
+    if (!graph && g->overlap && (mask & KIFMM_PHASE_U)) { /* the U-list ran on the side stream */
+      float mu = 0;
+      HIP_TRY(hipEventElapsedTime(&mu, g->su0, g->su1));
+      times->ulist = times->p2p_kernel = mu;
+    }
     times->copy = ms[10] + host_copy_ms;
     times->total = tot + host_copy_ms;
     if (graph) { /* one graph launch: no per-phase events inside it */
@@ -1148,6 +1177,15 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
       return KIFMM_EDEVICE;
     }
   }
+  g->overlap = opts ? (opts->flags & KIFMM_GPU_OVERLAP) != 0 : 0;
+  if (g->overlap && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+                     hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming) != hipSuccess ||
+                     hipEventCreate(&g->su0) != hipSuccess || hipEventCreate(&g->su1) != hipSuccess)) {
+    set_error("kifmm_gpu_create: second stream or its events could not be created");
+    kifmm_gpu_destroy(g);
+    return KIFMM_EDEVICE;
+  }
   try {
     r = build(g, tree, ops);
     if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED) r = mixed_setup(g);
@@ -1200,6 +1238,9 @@ int kifmm_gpu_destroy(kifmm_gpu* g) {
   for (void* p : g->allocs) (void)hipFree(p);
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : {g->fork_ev, g->join_ev, g->su0, g->su1})
+    if (e) (void)hipEventDestroy(e);
+  if (g->side) (void)hipStreamDestroy(g->side);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
   return KIFMM_OK;

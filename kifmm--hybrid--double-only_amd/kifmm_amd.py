@@ -26,6 +26,7 @@ PREC_FP64, PREC_MIXED = 0, 1
 GPU_GRAPH = 1
 GPU_NO_CLOSE_LIST = 2
 GPU_MIXED_INLINE = GPU_NO_CLOSE_LIST  # earlier name of the same flag
+GPU_OVERLAP = 4
 PHASE_UP, PHASE_U, PHASE_V, PHASE_W, PHASE_X, PHASE_DOWN, PHASE_ALL = 1, 2, 4, 8, 16, 32, 63
 
 
@@ -333,14 +334,16 @@ class GpuEvaluator:
     """Device copy of one problem (kifmm_gpu_create) and its evaluation."""
 
     def __init__(self, problem: Problem, device=0, precision=PREC_FP64, operators: Operators | None = None,
-                 graph=False, inline_close=False):
+                 graph=False, inline_close=False, overlap=False):
         """inline_close: do not list the close U-list pairs at create time;
         the mixed path then tests for them inside its loop and the FP64 path
-        forms r^2 from coordinate differences instead of the Gram form."""
+        forms r^2 from coordinate differences instead of the Gram form.
+        overlap: run the U-list on a second stream alongside up and V."""
         self.problem = problem
         self._view = problem.view()
         self._ops = operators if operators is not None else problem.operators()
-        opts = GpuOptions(device, precision, (GPU_GRAPH if graph else 0) | (GPU_NO_CLOSE_LIST if inline_close else 0))
+        opts = GpuOptions(device, precision, (GPU_GRAPH if graph else 0) | (GPU_NO_CLOSE_LIST if inline_close else 0)
+                          | (GPU_OVERLAP if overlap else 0))
         h = C.c_void_p()
         _check(lib.kifmm_gpu_create(C.byref(self._view), C.byref(self._ops), C.byref(opts), C.byref(h)))
         self._h = h

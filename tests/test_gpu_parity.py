@@ -146,3 +146,19 @@ def test_result_can_stay_on_device():
     assert none is None and t["copy"] < 0.5
     pot, _ = ev.evaluate(copy=True)
     assert np.array_equal(ev.intermediate("pot"), pot)
+
+
+@pytest.mark.parametrize("prec", [K.PREC_FP64, K.PREC_MIXED])
+def test_overlap_is_bitwise_identical(prec):
+    """KIFMM_GPU_OVERLAP (the U-list on a second stream alongside up and V,
+    joined before W, X and down) changes no sum: eager evaluations, graph
+    replays and a U-list-only phase mask equal the one-stream results."""
+    prob = K.Problem.create(60000, "gaussmix", 256, np_=4, seed=3)
+    one = K.GpuEvaluator(prob, precision=prec)
+    a = one.evaluate()[0]
+    u = one.evaluate_phases(K.PHASE_U)[0]
+    for graph in (False, True):
+        ev = K.GpuEvaluator(prob, precision=prec, overlap=True, graph=graph)
+        for _ in range(3):
+            assert np.array_equal(ev.evaluate()[0], a)
+        assert np.array_equal(ev.evaluate_phases(K.PHASE_U)[0], u)
