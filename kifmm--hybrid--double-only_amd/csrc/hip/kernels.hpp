@@ -1285,7 +1285,13 @@ struct Grid {
  * block: scatter 2^depth * uden onto the (2NP)^3 grid (only the NP^3 cube
  * at offset NP/2 is non-zero, so the three axis passes are pruned to it)
  * and produce the r2c spectrum [kz][ky][kx<=N/2] (FFTW layout), stored
- * interleaved by octant: X[blk][f][oct]. */
+ * interleaved by octant: X[blk][f][oct].  Each pass gives a thread one
+ * column: its NP inputs read once from LDS, all N (x pass: H) outputs of
+ * the column summed with the twiddles held in registers (the loops are
+ * unrolled, so every twiddle index is a constant).  The LDS planes are
+ * octant-fastest, [..][oct], so the lanes of a wave touch consecutive
+ * entries, and the z pass writes each kz plane of X contiguously.  Per
+ * output the sum runs over the input index in increasing order. */
 template <int NP, typename ST = double2>
 __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks, /* first child node per block */
                                                  const int* __restrict__ xslot,  /* X slot per block */
@@ -1296,63 +1302,81 @@ __global__ __launch_bounds__(512) void k_dft_fwd(const int* __restrict__ blocks,
                                                  ST* __restrict__ X) {
   using G = Grid<NP>;
   constexpr int N = G::N, H = G::H, F = G::F, NE = G::NE, OFF = G::OFF;
-  __shared__ double gs[8][NE];
-  __shared__ double2 A[8][NP][NP][H];  /* [oct][iz][iy][kx] */
-  __shared__ double2 B[8][NP][N][H];   /* [oct][iz][ky][kx] */
-  __shared__ double2 w[N];
+  __shared__ double gs[NE][8];
+  __shared__ double2 A[NP][NP][H][8]; /* [iz][iy][kx][oct] */
+  __shared__ double2 B[NP][N][H][8];  /* [iz][ky][kx][oct] */
   __shared__ int c2s[NP * NP * NP];
   const int tid = threadIdx.x;
   const int first = blocks[blockIdx.x];
-  if (tid < N) w[tid] = tw[tid];
+  double2 w[N];
+#pragma unroll
+  for (int m = 0; m < N; m++) w[m] = tw[m];
   for (int e = tid; e < NP * NP * NP; e += blockDim.x) c2s[e] = cube2surf[e];
   for (int e = tid; e < 8 * NE; e += blockDim.x) {
     const int o = e / NE, j = e % NE, node = first + o;
-    gs[o][j] = ldexp(uden[(size_t)node * ld_e + j], depth[node]);
+    gs[j][o] = ldexp(uden[(size_t)node * ld_e + j], depth[node]);
   }
   __syncthreads();
-  /* x pass: real -> half spectrum, input support x in [OFF, OFF+NP) */
-  for (int e = tid; e < 8 * NP * NP * H; e += blockDim.x) {
-    const int kx = e % H, iy = (e / H) % NP, iz = (e / (H * NP)) % NP, o = e / (H * NP * NP);
-    double re = 0.0, im = 0.0;
+  /* x pass: column (iz, iy, oct), real input x in [OFF, OFF + NP) -> kx <= N/2 */
+  for (int e = tid; e < NP * NP * 8; e += blockDim.x) {
+    const int o = e & 7, iy = (e >> 3) % NP, iz = (e >> 3) / NP;
+    double v[NP];
 #pragma unroll
     for (int ix = 0; ix < NP; ix++) {
       const int s = c2s[(ix * NP + iy) * NP + iz];
-      const double v = (s >= 0) ? gs[o][s] : 0.0;
-      const double2 t = w[(kx * (ix + OFF)) % N];
-      re = __builtin_fma(v, t.x, re);
-      im = __builtin_fma(v, t.y, im);
+      v[ix] = (s >= 0) ? gs[s][o] : 0.0;
     }
-    A[o][iz][iy][kx] = make_double2(re, im);
-  }
-  __syncthreads();
-  /* y pass */
-  for (int e = tid; e < 8 * NP * N * H; e += blockDim.x) {
-    const int kx = e % H, ky = (e / H) % N, iz = (e / (H * N)) % NP, o = e / (H * N * NP);
-    double re = 0.0, im = 0.0;
 #pragma unroll
-    for (int iy = 0; iy < NP; iy++) {
-      const double2 a = A[o][iz][iy][kx];
-      const double2 t = w[(ky * (iy + OFF)) % N];
-      re = __builtin_fma(a.x, t.x, __builtin_fma(-a.y, t.y, re));
-      im = __builtin_fma(a.x, t.y, __builtin_fma(a.y, t.x, im));
+    for (int kx = 0; kx < H; kx++) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int ix = 0; ix < NP; ix++) {
+        const double2 t = w[(kx * (ix + OFF)) % N];
+        re = __builtin_fma(v[ix], t.x, re);
+        im = __builtin_fma(v[ix], t.y, im);
+      }
+      A[iz][iy][kx][o] = make_double2(re, im);
     }
-    B[o][iz][ky][kx] = make_double2(re, im);
   }
   __syncthreads();
-  /* z pass, written coalesced as X[slot][f][oct] */
+  /* y pass: column (iz, kx, oct) */
+  for (int e = tid; e < NP * H * 8; e += blockDim.x) {
+    const int o = e & 7, kx = (e >> 3) % H, iz = (e >> 3) / H;
+    double2 a[NP];
+#pragma unroll
+    for (int iy = 0; iy < NP; iy++) a[iy] = A[iz][iy][kx][o];
+#pragma unroll
+    for (int ky = 0; ky < N; ky++) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int iy = 0; iy < NP; iy++) {
+        const double2 t = w[(ky * (iy + OFF)) % N];
+        re = __builtin_fma(a[iy].x, t.x, __builtin_fma(-a[iy].y, t.y, re));
+        im = __builtin_fma(a[iy].x, t.y, __builtin_fma(a[iy].y, t.x, im));
+      }
+      B[iz][ky][kx][o] = make_double2(re, im);
+    }
+  }
+  __syncthreads();
+  /* z pass: column (ky, kx, oct) = thread e; output (kz, ky, kx, oct) at
+   * X[slot][kz * N * H * 8 + e] */
   ST* out = X + (size_t)xslot[blockIdx.x] * F * 8;
-  for (int e = tid; e < 8 * F; e += blockDim.x) {
-    const int o = e & 7, f = e >> 3;
-    const int kx = f % H, ky = (f / H) % N, kz = f / (H * N);
-    double re = 0.0, im = 0.0;
+  for (int e = tid; e < N * H * 8; e += blockDim.x) {
+    const int o = e & 7, kk = e >> 3; /* kk = ky * H + kx */
+    double2 b[NP];
 #pragma unroll
-    for (int iz = 0; iz < NP; iz++) {
-      const double2 b = B[o][iz][ky][kx];
-      const double2 t = w[(kz * (iz + OFF)) % N];
-      re = __builtin_fma(b.x, t.x, __builtin_fma(-b.y, t.y, re));
-      im = __builtin_fma(b.x, t.y, __builtin_fma(b.y, t.x, im));
+    for (int iz = 0; iz < NP; iz++) b[iz] = B[iz][kk / H][kk % H][o];
+#pragma unroll
+    for (int kz = 0; kz < N; kz++) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int iz = 0; iz < NP; iz++) {
+        const double2 t = w[(kz * (iz + OFF)) % N];
+        re = __builtin_fma(b[iz].x, t.x, __builtin_fma(-b[iz].y, t.y, re));
+        im = __builtin_fma(b[iz].x, t.y, __builtin_fma(b[iz].y, t.x, im));
+      }
+      out[(size_t)kz * N * H * 8 + e] = Spec<ST>::mk(re, im);
     }
-    out[e] = Spec<ST>::mk(re, im);
   }
 }
 
