@@ -1712,7 +1712,11 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
 /* Inverse transform of the 8 sibling spectra of one child block, pruned to
  * the NP^3 cube holding the check surface; normalised by 1/N^3 and added
  * into the downward check values (compute_ifft_trg, fmm/omp_sse_block.cc:482-533:
- * FFTW c2r semantics, the kx = 0 and kx = N/2 imaginary parts are ignored). */
+ * FFTW c2r semantics, the kx = 0 and kx = N/2 imaginary parts are ignored).
+ * The z and y passes give a thread one column (inputs read once, all NP
+ * outputs with the twiddles in registers, unrolled); the LDS planes are
+ * octant-fastest.  Per output the sum runs over the input index in
+ * increasing order. */
 template <int NP, typename ST = double2>
 __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks, const int* __restrict__ yslot,
                                                  const ST* __restrict__ Y, const double2* __restrict__ tw,
@@ -1720,22 +1724,25 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
                                                  double* __restrict__ dchk, int ld_e) {
   using G = Grid<NP>;
   constexpr int N = G::N, H = G::H, F = G::F, NE = G::NE, OFF = G::OFF;
-  __shared__ double2 B[8][NP][N][H]; /* [oct][iz][ky][kx] */
-  __shared__ double2 A[8][NP][NP][H]; /* [oct][iz][iy][kx] */
-  __shared__ double2 w[N];
+  __shared__ double2 B[NP][N][H][8];  /* [iz][ky][kx][oct] */
+  __shared__ double2 A[NP][NP][H][8]; /* [iz][iy][kx][oct] */
+  __shared__ double2 ws[N];           /* for the x pass, whose twiddle index is not a constant */
   const int tid = threadIdx.x;
   const int first = blocks[blockIdx.x];
-  if (tid < N) w[tid] = tw[tid];
-  __syncthreads();
+  double2 w[N];
+#pragma unroll
+  for (int m = 0; m < N; m++) w[m] = tw[m];
+  if (tid < N) ws[tid] = tw[tid];
   const ST* in = Y + (size_t)yslot[blockIdx.x] * F * 8;
-  /* z pass: thread per (ky, kx, oct), NP outputs in registers */
+  /* z pass: column (ky, kx, oct) = thread e, reading Y[slot][kz * N * H * 8 + e] */
   for (int e = tid; e < N * H * 8; e += blockDim.x) {
     const int o = e & 7, kk = e >> 3; /* kk = ky*H + kx */
     double re[NP], im[NP];
 #pragma unroll
     for (int iz = 0; iz < NP; iz++) { re[iz] = 0.0; im[iz] = 0.0; }
+#pragma unroll
     for (int kz = 0; kz < N; kz++) {
-      const ST vs = in[(size_t)(kz * N * H + kk) * 8 + o];
+      const ST vs = in[(size_t)kz * N * H * 8 + e];
       const double2 v = make_double2(vs.x, vs.y);
 #pragma unroll
       for (int iz = 0; iz < NP; iz++) {
@@ -1746,20 +1753,26 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
     }
     const int ky = kk / H, kx = kk % H;
 #pragma unroll
-    for (int iz = 0; iz < NP; iz++) B[o][iz][ky][kx] = make_double2(re[iz], im[iz]);
+    for (int iz = 0; iz < NP; iz++) B[iz][ky][kx][o] = make_double2(re[iz], im[iz]);
   }
   __syncthreads();
-  /* y pass */
-  for (int e = tid; e < 8 * NP * NP * H; e += blockDim.x) {
-    const int kx = e % H, iy = (e / H) % NP, iz = (e / (H * NP)) % NP, o = e / (H * NP * NP);
-    double re = 0.0, im = 0.0;
-    for (int ky = 0; ky < N; ky++) {
-      const double2 v = B[o][iz][ky][kx];
-      const double2 t = w[(ky * (iy + OFF)) % N];
-      re = __builtin_fma(v.x, t.x, __builtin_fma(v.y, t.y, re));
-      im = __builtin_fma(v.y, t.x, __builtin_fma(-v.x, t.y, im));
+  /* y pass: column (iz, kx, oct) */
+  for (int e = tid; e < NP * H * 8; e += blockDim.x) {
+    const int o = e & 7, kx = (e >> 3) % H, iz = (e >> 3) / H;
+    double2 v[N];
+#pragma unroll
+    for (int ky = 0; ky < N; ky++) v[ky] = B[iz][ky][kx][o];
+#pragma unroll
+    for (int iy = 0; iy < NP; iy++) {
+      double re = 0.0, im = 0.0;
+#pragma unroll
+      for (int ky = 0; ky < N; ky++) {
+        const double2 t = w[(ky * (iy + OFF)) % N];
+        re = __builtin_fma(v[ky].x, t.x, __builtin_fma(v[ky].y, t.y, re));
+        im = __builtin_fma(v[ky].y, t.x, __builtin_fma(-v[ky].x, t.y, im));
+      }
+      A[iz][iy][kx][o] = make_double2(re, im);
     }
-    A[o][iz][iy][kx] = make_double2(re, im);
   }
   __syncthreads();
   /* x pass (c2r) at the surface samples */
@@ -1768,13 +1781,14 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
     const int o = e / NE, j = e % NE;
     const int cube = surf2cube[j];
     const int ix = cube / (NP * NP), iy = (cube / NP) % NP, iz = cube % NP;
-    const double2* r = A[o][iz][iy];
+    const double2 (*r)[8] = A[iz][iy];
     const int xg = ix + OFF;
-    double s = r[0].x + ((xg & 1) ? -r[N / 2].x : r[N / 2].x);
+    double s = r[0][o].x + ((xg & 1) ? -r[N / 2][o].x : r[N / 2][o].x);
     double acc = 0.0;
+#pragma unroll
     for (int kx = 1; kx < N / 2; kx++) {
-      const double2 t = w[(kx * xg) % N];
-      acc = __builtin_fma(r[kx].x, t.x, __builtin_fma(r[kx].y, t.y, acc));
+      const double2 t = ws[(kx * xg) % N];
+      acc = __builtin_fma(r[kx][o].x, t.x, __builtin_fma(r[kx][o].y, t.y, acc));
     }
     s = __builtin_fma(2.0, acc, s);
     dchk[(size_t)(first + o) * ld_e + j] += nrm * s;
