@@ -931,52 +931,61 @@ __global__ __launch_bounds__(128) void k_close_pairs(const int4* __restrict__ it
 
 /* ------------------------------------------------------------------ S2M */
 
-/* Lanes as targets, sources wave-uniform.  Every lane of the wave walks the
- * same n sources (x, y, z, w), which the compiler reads with scalar loads
- * and feeds to the VALU as SGPR operands: no LDS, no barrier, and no
- * cross-lane reduction (each lane owns its sum).  Used where the sources of
- * a whole wave are one short list: a leaf's points (S2M) or a box's
- * equivalent surface (L2T).  Returns the sum of 2 w / r, in a fixed
- * summation order (deterministic). */
-template <bool MASK>
-__device__ __forceinline__ double sum_uniform(double tx, double ty, double tz, const double4* __restrict__ src,
-                                              int n) {
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
-  int k = 0;
-  for (; k + 4 <= n; k += 4) { /* four sources per step: one batch of scalar loads */
-#pragma unroll
-    for (int u = 0; u < 4; u++) {
-      const double4 s0 = src[k + u];
-      const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-      acc[u] = __builtin_fma(s0.w, inv_r2<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[u]);
-    }
-  }
-  for (; k < n; k++) {
-    const double4 s0 = src[k];
-    const double ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-    acc[0] = __builtin_fma(s0.w, inv_r2<MASK>(__builtin_fma(az, az, __builtin_fma(ay, ay, ax * ax))), acc[0]);
-  }
-  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
-}
-
 /* S2M check potentials (up_eval__gpu, fmm/cuda.cu:108-190 first half): one
- * block per source leaf, lane j of the block is UC check point j at
- * c + r * SP_UC[j]; the leaf's sources are wave-uniform.  Block size is
- * n_uc rounded up to whole waves (at most 320 for NP = 6). */
+ * block per source leaf, lane j of the block is UC check point j.  In leaf
+ * units (centre c, radius h of the node: s' = (s - c) / h, a power-of-two
+ * scale, exact) the check surface is the same table for every leaf (rows
+ * x, y, z, |p|^2 of the radius-3 box), and the leaf's sources, staged
+ * S2M_TILE at a time in LDS as (s', |s'|^2) and w, are read as wave-wide
+ * broadcasts.  Per pair the Gram form
+ *     r'^2 = (|p|^2 + |s'|^2) + (-2p) . s'        (1 add, 3 FMA)
+ * then the rsq, the Newton step and the FMA: 9 VALU instructions against
+ * 11 for coordinate differences.  The check points lie at least 2 leaf
+ * units from the leaf's points (|p| <= 3 sqrt(3), |s'| <= sqrt(3)), so the
+ * Gram r'^2 is within a few eps (|p| + |s'|)^2 / 4 relative: ~1e-15.
+ * Block size is n_uc rounded up to whole waves (at most 320 for NP = 6). */
+constexpr int S2M_TILE = 512;
 static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
                                              const double4* __restrict__ geo, /* node centre xyz + radius */
-                                             const double* __restrict__ surf_uc, /* 3 x ld, unit surface */
+                                             const double* __restrict__ surf_uc, /* 4 x ld, radius-3 box */
                                              int n_uc, int ld_uc_surf,
                                              const double4* __restrict__ src4,
                                              double* __restrict__ chk, int ld_chk) {
+  __shared__ double4 sS[S2M_TILE];
+  __shared__ double sW[S2M_TILE];
   const int4 it = items[blockIdx.x];
   const int j = threadIdx.x, jj = min(j, n_uc - 1);
   const double4 g = geo[it.x];
-  const double tx = g.x + g.w * surf_uc[jj];
-  const double ty = g.y + g.w * surf_uc[ld_uc_surf + jj];
-  const double tz = g.z + g.w * surf_uc[2 * ld_uc_surf + jj];
-  const double acc = sum_uniform<false>(tx, ty, tz, src4 + it.y, it.z);
-  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = HALF_OOFP_R * acc;
+  const double ih = 1.0 / g.w;
+  const double ax = -2.0 * surf_uc[jj], ay = -2.0 * surf_uc[ld_uc_surf + jj], az = -2.0 * surf_uc[2 * ld_uc_surf + jj];
+  const double pp = surf_uc[3 * ld_uc_surf + jj];
+  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int base = 0; base < it.z; base += S2M_TILE) {
+    const int n = min(S2M_TILE, it.z - base);
+    __syncthreads(); /* the previous tile's readers are done */
+    for (int k = j; k < n; k += blockDim.x) {
+      const double4 q = src4[it.y + base + k];
+      const double x = (q.x - g.x) * ih, y = (q.y - g.y) * ih, z = (q.z - g.z) * ih;
+      sS[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+      sW[k] = q.w;
+    }
+    __syncthreads();
+    int k = 0;
+    for (; k + 4 <= n; k += 4) {
+#pragma unroll
+      for (int u = 0; u < 4; u++) {
+        const double4 q = sS[k + u];
+        const double r2 = __builtin_fma(ax, q.x, __builtin_fma(ay, q.y, __builtin_fma(az, q.z, pp + q.w)));
+        acc[u] = __builtin_fma(sW[k + u], rsqrt2_fp64(r2), acc[u]);
+      }
+    }
+    for (; k < n; k++) {
+      const double4 q = sS[k];
+      const double r2 = __builtin_fma(ax, q.x, __builtin_fma(ay, q.y, __builtin_fma(az, q.z, pp + q.w)));
+      acc[0] = __builtin_fma(sW[k], rsqrt2_fp64(r2), acc[0]);
+    }
+  }
+  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = (HALF_OOFP_R * ih) * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
 }
 
 /* Mixed-precision S2M and L2T (NP <= 4): the same lane-per-target scheme
