@@ -7,7 +7,8 @@
  * targets), SEED (default 26, the reference's srand48(26)).
  * Additions: distribution "file:<path>" reads a KIFMMPT1 point file (the
  * count argument is then ignored); PREC=mixed selects the mixed-precision
- * path (NP <= 4); GRAPH=1 replays the evaluation as one HIP graph.
+ * path (NP <= 4); GRAPH=1 replays the evaluation as one HIP graph;
+ * OVERLAP=1 runs the U-list on a second stream alongside up and V.
  * Steps: generate + trees + lists + gather, operators, kifmm_gpu_create
  * (preproc), kifmm_gpu_evaluate (run, with the copy back timed),
  * scatter to input order, validate, print the reference's estimates.
@@ -58,6 +59,8 @@ int main(int argc, char** argv) {
   if (prec && strcmp(prec, "mixed") == 0) go.precision = KIFMM_PREC_MIXED;
   const char* graph = getenv("GRAPH");
   if (graph && atoi(graph) != 0) go.flags |= KIFMM_GPU_GRAPH;
+  const char* overlap = getenv("OVERLAP");
+  if (overlap && atoi(overlap) != 0) go.flags |= KIFMM_GPU_OVERLAP;
   kifmm_gpu* g = nullptr;
   if (kifmm_gpu_create(&tv, &ops, &go, &g) != KIFMM_OK) {
     fprintf(stderr, "error: %s\n", kifmm_last_error());

@@ -1,0 +1,36 @@
+"""The fmmd--hip command line, the drop-in for fmmd--cuda (fmm/driver.cc:211-373):
+exit status, the reference's report lines, and the sampled relative error
+(NV = 200 samples, validate of fmm/driver.cc:141-206) at NP = 4 under each
+environment option.  NP = 4 with 64 points per box stays below 1e-4 on
+every path (the FP64 and mixed paths agree to ~1e-7 on it)."""
+import os
+import re
+import subprocess
+
+import pytest
+
+import kifmm_amd as K
+
+pytestmark = pytest.mark.gpu
+
+EXE = K.HERE / "build" / "fmmd--hip"
+
+
+@pytest.mark.parametrize("env", [{}, {"PREC": "mixed"}, {"GRAPH": "1"}, {"OVERLAP": "1", "C": "1"},
+                                 {"PREC": "mixed", "GRAPH": "1", "OVERLAP": "1"}],
+                         ids=["fp64", "mixed", "graph", "overlap-coincident", "mixed-graph-overlap"])
+def test_fmmd_hip_runs_and_validates(env):
+    e = dict(os.environ, NP="4", NV="200", **env)
+    print(f"case fmmd--hip 65536 uniform 64 {env}", flush=True)
+    r = subprocess.run([str(EXE), "65536", "uniform", "64"], env=e, capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr)
+    assert r.returncode == 0
+    assert "Code: hip-gfx950" in r.stdout and "Inputs: 65536 | uniform | 64" in r.stdout
+    assert "==> Total Execution Time:" in r.stderr
+    m = re.search(r"Relative error: (\S+)", r.stderr)
+    assert m and 0.0 < float(m.group(1)) < 1e-4
+
+
+def test_fmmd_hip_rejects_bad_arguments():
+    r = subprocess.run([str(EXE), "1000"], capture_output=True, text=True, timeout=60)
+    assert r.returncode != 0
