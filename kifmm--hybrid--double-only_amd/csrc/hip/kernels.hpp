@@ -108,24 +108,6 @@ __device__ __forceinline__ void eval8_lds(const double4* __restrict__ src, int n
   }
 }
 
-/* Loads n points of a list into the LDS tile (n <= cap, padded to a
- * multiple of 64 with zero-weight far points).  Either gathered sources
- * (surf == nullptr) or a surface c + r * SP with densities den. */
-__device__ __forceinline__ int stage_sources(double4* tile, int cap, const double4* __restrict__ src4, int beg,
-                                             int n, const double* __restrict__ surf, int ld_surf, double4 g,
-                                             const double* __restrict__ den, int tid, int nthreads) {
-  const int npad = min(cap, ((n + 63) / 64) * 64);
-  for (int k = tid; k < npad; k += nthreads) {
-    double4 v = make_double4(FAR, FAR, FAR, 0.0);
-    if (k < n) {
-      if (surf) v = make_double4(g.x + g.w * surf[k], g.y + g.w * surf[ld_surf + k], g.z + g.w * surf[2 * ld_surf + k], den[k]);
-      else v = src4[beg + k];
-    }
-    tile[k] = v;
-  }
-  return npad;
-}
-
 /* ------------------------------------------------------------------ U-list */
 
 /* U-list work item = {source range set, first target, target count, leaf},
@@ -134,8 +116,9 @@ __device__ __forceinline__ int stage_sources(double4* tile, int cap, const doubl
  * up to G groups of accumulators and every staged source tile serves all
  * the item's targets.  The sources are the leaf's U-list, a list of merged
  * ranges {beg, num} of the gathered source array, walked in order.  (The 8
- * targets of a group are wave-uniform and live in SGPRs: the inner loop is
- * 3 sub + 3 for r^2, v_rsq_f64 + 3 for the Newton step, 1 FMA per pair.)
+ * targets of a group are wave-uniform and live in SGPRs.  Per pair the
+ * difference form spends 3 sub + 3 for r^2, v_rsq_f64 + 3 for the Newton
+ * step and 1 FMA; the Gram form, below, 1 add + 3 FMA for r^2.)
  * Per target, the sum is the same for every item size and launch form:
  * the tiles depend only on the ranges, the lane order and the reduction
  * are fixed. */
