@@ -1,7 +1,8 @@
 """Randomized parity sweep: 24 small inputs drawn with a fixed seed across
 every generator, points per box from 1 to 200, NP 3 / 4 / 6, separate and
 coincident targets, each evaluated under a randomly drawn launch shape
-(U-list form, Hadamard form and frequency block).  The FP64 potentials
+(U-list form, Hadamard form and frequency block) and stream layout
+(one stream, or the U-list overlapped on a second one).  The FP64 potentials
 must meet the parity bar against the oracle (oracle_lib.PARITY_RTOL); for
 NP <= 4 the mixed path must be within 1e-5 of the oracle and finite.  The
 fixed cases of test_gpu_parity.py pin specific shapes; this one looks for
@@ -26,7 +27,7 @@ def draw_cases(count=24, seed=2026):
         n = int(np.exp(rng.uniform(np.log(300), np.log(nmax))))
         out.append((n, DISTS[i % 4], q, int(rng.choice([3, 4, 6])), int(rng.integers(0, 1000)),
                     int(rng.integers(0, 2)), int(rng.integers(0, 3)), int(rng.choice([4, 8, 16])),
-                    int(rng.integers(0, 2))))
+                    int(rng.integers(0, 2)), bool(rng.integers(0, 2))))
     return out
 
 
@@ -36,9 +37,9 @@ def rel(a, b):
 
 @pytest.mark.parametrize("case", draw_cases(), ids=lambda c: f"{c[1]}-n{c[0]}-q{c[2]}-np{c[3]}-C{c[5]}")
 def test_random_inputs_and_shapes_match_oracle(case):
-    n, dist, q, np_, seed, c, form, fb, had = case
-    print(f"case n={n} dist={dist} q={q} NP={np_} seed={seed} C={c} p2p_form={form} fb={fb} had={had}",
-          flush=True)
+    n, dist, q, np_, seed, c, form, fb, had, ovl = case
+    print(f"case n={n} dist={dist} q={q} NP={np_} seed={seed} C={c} p2p_form={form} fb={fb} had={had} "
+          f"overlap={ovl}", flush=True)
     prob = K.Problem.create(n, dist, q, np_=np_, seed=seed, coincide=c)
     o = prob.operators()
     orc = O.Oracle(n, dist, q, np_, seed, c)
@@ -46,14 +47,14 @@ def test_random_inputs_and_shapes_match_oracle(case):
                         ((o.uc2ue, o.n_e * o.n_uc), (o.ue2uc, 8 * o.n_uc * o.n_e), (o.dc2de, o.n_e ** 2),
                          (o.de2dc, 8 * o.n_e ** 2))])
     ref, _ = orc.evaluate(O.KERNEL_EXACT)
-    ev = K.GpuEvaluator(prob)
+    ev = K.GpuEvaluator(prob, overlap=ovl)
     ev.set_tuning(form, fb, had)
     got = prob.scatter(ev.evaluate()[0])
     r = rel(got, ref)
     print(f"  FP64 vs oracle {r:.2e}", flush=True)
     assert r < O.PARITY_RTOL[np_]
     if np_ <= 4:
-        mx = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+        mx = K.GpuEvaluator(prob, precision=K.PREC_MIXED, overlap=ovl)
         mx.set_tuning(K.P2P_STATIC, fb, had)
         pm = prob.scatter(mx.evaluate()[0])
         rm = rel(pm, ref)
