@@ -265,7 +265,7 @@ def main():
     OVERLAP = args.overlap
     if args.gpus != 1:
         print(f"bench.py: this build evaluates on one GPU; --gpus {args.gpus} is not supported "
-              f"(the multi-GPU sharding is a design in DESIGN.md, not a code path).", file=sys.stderr)
+              f"(how the evaluation would shard over several GPUs is a written design, not a code path).", file=sys.stderr)
         return 2
     if args.calibrate_copy:
         return calibrate_copy(args.calibrate_copy)
