@@ -125,6 +125,12 @@ __device__ __forceinline__ void eval8_lds(const double4* __restrict__ src, int n
 constexpr int P2P_TILE = 512;
 constexpr int P2P_G = 4;
 constexpr int P2P_MAXT = 4 * 8 * P2P_G;
+/* The FP64 Gram-form kernels take items of up to 160 targets (5 groups per
+ * wave, 120 VGPRs: still 4 waves per SIMD): a leaf of 129-160 targets is
+ * then one item instead of two, which saves a source walk and evens out
+ * the groups per wave. */
+constexpr int P2P_G5 = 5;
+constexpr int P2P_MAXT5 = 4 * 8 * P2P_G5;
 
 /* Walks an item's source ranges {beg, num} one tile (P2P_TILE sources) at a
  * time, in order.  The walk state is block-uniform; thread tid fetches the
@@ -365,7 +371,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
                                                    const int* __restrict__ cptr, const int* __restrict__ cidx) {
   __shared__ double4 tile[P2P_TILE];
   __shared__ double sw[GRAM ? P2P_TILE : 1];
-  __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
+  __shared__ double sc_sum[GRAM ? 32 * G : 1];
   /* workgroups go round-robin to the 8 XCDs: give each XCD a contiguous run
    * of items (neighbouring leaves) so their shared sources stay in its L2 */
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
@@ -395,12 +401,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
   p2p_item<MASK, P2P_G / 2, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
 }
 
-/* Persistent form over the same items (<= P2P_MAXT targets): a fixed grid
+/* Persistent form over the same items (<= 32 G targets): a fixed grid
  * of a few blocks per CU; the blocks of XCD x draw the items of x's
  * contiguous run one at a time from counter x (zeroed before the launch),
  * so uneven items even out inside each XCD.  Every block leaves the loop
  * at its first draw past the run's end; the counters only grow. */
-template <bool MASK, bool GRAM>
+template <bool MASK, int G, bool GRAM>
 __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restrict__ items, int n_items, int per_xcd,
                                                            const int* __restrict__ rng_ptr,
                                                            const int2* __restrict__ rng,
@@ -412,7 +418,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restric
                                                            const int* __restrict__ cptr, const int* __restrict__ cidx) {
   __shared__ double4 tile[P2P_TILE];
   __shared__ double sw[GRAM ? P2P_TILE : 1];
-  __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
+  __shared__ double sc_sum[GRAM ? 32 * G : 1];
   __shared__ int draw;
   const int x = (int)(blockIdx.x & 7);
   const int lo = x * per_xcd, n = min(n_items - lo, per_xcd);
@@ -423,7 +429,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restric
     const int k = draw;
     __syncthreads(); /* every thread has read draw before the next draw overwrites it */
     if (k >= n) break;
-    p2p_item<MASK, P2P_G, GRAM>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot, ga);
+    p2p_item<MASK, G, GRAM>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot, ga);
   }
 }
 

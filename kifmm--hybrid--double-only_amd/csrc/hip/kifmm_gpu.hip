@@ -85,6 +85,7 @@ struct kifmm_gpu {
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
+  int4* p2p_items5 = nullptr; int n_p2p5 = 0; /* <= P2P_MAXT5 targets: the FP64 Gram-form kernels */
   int* rng_ptr = nullptr; int2* rng = nullptr;
   int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
   int persist_grid[2] = {0, 0}; /* blocks of the persistent U-list form: difference, Gram */
@@ -243,7 +244,7 @@ struct HostLevel {
 struct Plan {
   int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
   int N = 0, H = 0, F = 0, nblk = 0, max_depth = 0;
-  std::vector<int4> p2p_items, p2p_items64;
+  std::vector<int4> p2p_items, p2p_items64, p2p_items5;
   std::vector<int> rng_ptr;
   std::vector<int2> rng;
   std::vector<int4> s2m_items;
@@ -295,7 +296,8 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
   };
 
   /* U-list: merged source ranges per target leaf; a leaf's targets split
-   * into the fewest items of <= P2P_MAXT, balanced, in multiples of 16 */
+   * into the fewest items of <= P2P_MAXT, balanced, in multiples of 16
+   * (also of <= P2P_MAXT / 2 and <= P2P_MAXT5 for the other launch shapes) */
   P.rng_ptr.assign(1, 0);
   for (int i = 0; i < nn; i++) {
     if (t->child[i] != -1 || t->trg_num[i] <= 0) continue;
@@ -310,16 +312,16 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     }
     P.rng_ptr.push_back((int)P.rng.size());
     const int n = t->trg_num[i];
-    for (int cap : {P2P_MAXT, P2P_MAXT / 2}) { /* the item lists of both launch shapes */
+    for (int cap : {P2P_MAXT, P2P_MAXT / 2, P2P_MAXT5}) { /* the item lists of the launch shapes */
       const int nit = (n + cap - 1) / cap;
       const int per = (((n + nit - 1) / nit) + 15) & ~15;
-      std::vector<int4>& out = cap == P2P_MAXT ? P.p2p_items : P.p2p_items64;
+      std::vector<int4>& out = cap == P2P_MAXT ? P.p2p_items : cap == P2P_MAXT5 ? P.p2p_items5 : P.p2p_items64;
       for (int c = 0; c < n; c += per) out.push_back(make_int4(set, t->trg_beg[i] + c, std::min(per, n - c), i));
     }
   }
   /* k_p2p_ulist reads trg4[it.y .. it.y+it.z-1] and src4 inside each range */
-  for (int cap : {P2P_MAXT, P2P_MAXT / 2})
-    for (const int4& it : cap == P2P_MAXT ? P.p2p_items : P.p2p_items64)
+  for (int cap : {P2P_MAXT, P2P_MAXT / 2, P2P_MAXT5})
+    for (const int4& it : cap == P2P_MAXT ? P.p2p_items : cap == P2P_MAXT5 ? P.p2p_items5 : P.p2p_items64)
       if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > cap || it.y < 0 ||
           (long)it.y + it.z > t->n_trg || it.w < 0 || it.w >= nn)
         return fail("U-list work item out of range");
@@ -675,13 +677,15 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   TRY(dev_upload(g, &g->p2p_items, P.p2p_items.data(), P.p2p_items.size()));
   g->n_p2p64 = (int)P.p2p_items64.size();
   TRY(dev_upload(g, &g->p2p_items64, P.p2p_items64.data(), P.p2p_items64.size()));
+  g->n_p2p5 = (int)P.p2p_items5.size();
+  TRY(dev_upload(g, &g->p2p_items5, P.p2p_items5.data(), P.p2p_items5.size()));
   TRY(dev_zero(g, &g->p2p_counters, 8));
   {
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev));
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, false>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, P2P_G, false>, 256, 0));
     g->persist_grid[0] = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
-    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, true>, 256, 0));
+    HIP_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, k_p2p_ulist_persist<false, P2P_G5, true>, 256, 0));
     g->persist_grid[1] = std::max(8, (ncu * std::max(per_cu, 1) + 7) / 8 * 8);
   }
   TRY(dev_upload(g, &g->rng_ptr, P.rng_ptr.data(), P.rng_ptr.size()));
@@ -880,12 +884,18 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   return vlist_phase<NP, double2>(g, g->X, g->Y, g->T, e1, e2, e3);
 }
 
+/* FP64 U-list launch: the Gram form on items of <= P2P_MAXT5 targets, the
+ * difference form (no close-pair list) on items of <= P2P_MAXT; the
+ * STATIC64 shape on items of <= 64 either way. */
 template <bool MASK, bool GRAM>
-int p2p_fp64(kifmm_gpu* g, int per_xcd) {
+int p2p_fp64(kifmm_gpu* g) {
+  constexpr int G = GRAM ? P2P_G5 : P2P_G;
+  const int4* items = GRAM ? g->p2p_items5 : g->p2p_items;
+  const int n_items = GRAM ? g->n_p2p5 : g->n_p2p, per_xcd = (n_items + 7) / 8;
   if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
     hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, g->stream, g->p2p_counters, 8, 0);
-    hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, GRAM>), dim3(g->persist_grid[GRAM]), dim3(256), 0, g->stream,
-                       g->p2p_items, g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters,
+    hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, G, GRAM>), dim3(g->persist_grid[GRAM]), dim3(256), 0, g->stream,
+                       items, n_items, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters,
                        g->geo, g->trg_gram, g->cptr, g->cidx);
   } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
     const int per64 = (g->n_p2p64 + 7) / 8;
@@ -893,8 +903,8 @@ int p2p_fp64(kifmm_gpu* g, int per_xcd) {
                        g->p2p_items64, g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo,
                        g->trg_gram, g->cptr, g->cidx);
   } else {
-    hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G, GRAM>), dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
-                       g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo, g->trg_gram, g->cptr,
+    hipLaunchKernelGGL((k_p2p_ulist<MASK, G, GRAM>), dim3(8 * per_xcd), dim3(256), 0, g->stream, items, n_items,
+                       per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo, g->trg_gram, g->cptr,
                        g->cidx);
   }
   LAUNCH_CHECK();
@@ -913,9 +923,9 @@ int point_phases_u(kifmm_gpu* g) {
       hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
                          g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
     else if (g->trg_gram)
-      return p2p_fp64<MASK, true>(g, per_xcd);
+      return p2p_fp64<MASK, true>(g);
     else
-      return p2p_fp64<MASK, false>(g, per_xcd);
+      return p2p_fp64<MASK, false>(g);
   }
   LAUNCH_CHECK();
   return KIFMM_OK;
