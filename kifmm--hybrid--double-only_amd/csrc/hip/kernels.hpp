@@ -727,8 +727,12 @@ __device__ __forceinline__ void mixed_tau_chunks(const v8bf (*sB)[P2P_TILE], con
     for (int e = 0; e < 4; e++) acc[q][e] += (double)part[q][e];
 }
 
+/* Groups of 16 targets per wave: items of up to P2P_MAXT5 targets (10
+ * groups) give a wave at most 3. */
+constexpr int MIX_G = 3;
+static_assert(4 * 16 * MIX_G >= P2P_MAXT5, "mixed U-list groups");
 template <bool MASK>
-__global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restrict__ items, int n_items, int per_xcd,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_p2p_ulist_mixed_tau(const int4* __restrict__ items, int n_items, int per_xcd,
                                                              const int* __restrict__ rng_ptr,
                                                              const int2* __restrict__ rng,
                                                              const double4* __restrict__ geo,
@@ -739,7 +743,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
                                                              double* __restrict__ pot) {
   __shared__ v8bf sB[4][P2P_TILE];
   __shared__ float sW[P2P_TILE];
-  __shared__ double4 sT[P2P_MAXT];
+  __shared__ double4 sT[P2P_MAXT5];
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
   const int4 it = items[item];
@@ -755,11 +759,11 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
   }
   __syncthreads();
-  v8bf a[2];
-  v4f c[2];
-  double acc[2][4];
+  v8bf a[MIX_G];
+  v4f c[MIX_G];
+  double acc[MIX_G][4];
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
+  for (int q = 0; q < MIX_G; q++) {
     const int grp = min(wave + 4 * q, ngroups - 1);
     const double4 t = sT[16 * grp + (lane & 15)];
     const int k4 = lane >> 4;
@@ -792,17 +796,22 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
     }
     __syncthreads();
     const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
-    if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, ns >> 4, a, c, acc, lane);
+    if (wave + 8 < ngroups) mixed_tau_chunks<3>(sB, sW, ns >> 4, a, c, acc, lane);
+    else if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, ns >> 4, a, c, acc, lane);
     else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, ns >> 4, a, c, acc, lane);
     fill = next;
   }
   /* FP64 close-pair terms of the wave's target groups.  A group's pairs are
    * one contiguous run of cidx (targets in order), spread over the 64 lanes
    * whatever their distribution over the 16 targets, then summed per target
-   * by a segmented scan over the lanes (fixed order: deterministic). */
-  __shared__ double sC[P2P_MAXT];
+   * by a segmented scan over the lanes (fixed order: deterministic).  The
+   * per-target sums reuse the source tile's LDS once every wave is done
+   * with it (keeps the block within 40 KB: 4 blocks per CU). */
+  static_assert(sizeof(sB) >= P2P_MAXT5 * sizeof(double), "close-pair sums in the tile");
+  __syncthreads();
+  double* const sC = reinterpret_cast<double*>(&sB[0][0]);
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
+  for (int q = 0; q < MIX_G; q++) {
     const int grp = wave + 4 * q;
     if (grp < ngroups) {
       const int t0 = 16 * grp, nt = min(16, it.z - t0), tg0 = it.y + t0;
@@ -837,7 +846,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed_tau(const int4* __restr
   }
   const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
 #pragma unroll
-  for (int q = 0; q < 2; q++) {
+  for (int q = 0; q < MIX_G; q++) {
     const int grp = wave + 4 * q;
     if (grp < ngroups) {
 #pragma unroll

@@ -85,7 +85,7 @@ struct kifmm_gpu {
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
-  int4* p2p_items5 = nullptr; int n_p2p5 = 0; /* <= P2P_MAXT5 targets: the FP64 Gram-form kernels */
+  int4* p2p_items5 = nullptr; int n_p2p5 = 0; /* <= P2P_MAXT5 targets: the FP64 Gram-form and mixed tau kernels */
   int* rng_ptr = nullptr; int2* rng = nullptr;
   int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
   int persist_grid[2] = {0, 0}; /* blocks of the persistent U-list form: difference, Gram */
@@ -915,10 +915,11 @@ template <bool MASK>
 int point_phases_u(kifmm_gpu* g) {
   if (g->n_p2p > 0) {
     const int per_xcd = (g->n_p2p + 7) / 8;
-    if (g->precision == KIFMM_PREC_MIXED && g->close_listed)
-      hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
-                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx,
-                         g->pot);
+    if (g->precision == KIFMM_PREC_MIXED && g->close_listed) {
+      const int per5 = (g->n_p2p5 + 7) / 8; /* items of <= P2P_MAXT5 targets */
+      hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per5), dim3(256), 0, g->stream, g->p2p_items5,
+                         g->n_p2p5, per5, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx, g->pot);
+    }
     else if (g->precision == KIFMM_PREC_MIXED)
       hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
                          g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
