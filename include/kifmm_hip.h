@@ -81,7 +81,8 @@ typedef struct kifmm_phase_times {
   double ulist;     /* U-list P2P                         */
   double vlist;     /* sum of the four V sub-phases       */
   double v_fft_src; /* surface -> grid scatter + forward DFT */
-  double v_trans;   /* translation spectra (316 vectors)  */
+  double v_trans;   /* translation spectra (316 vectors): formed once at
+                       create time, so ~0 here               */
   double v_hadamard;/* spectral multiply-accumulate       */
   double v_ifft;    /* inverse DFT + check-surface gather */
   double wlist;
