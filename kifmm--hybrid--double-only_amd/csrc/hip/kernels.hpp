@@ -204,6 +204,7 @@ __device__ __forceinline__ double gram_pair(double4 a, double4 s) {
 /* acc[t] += w * gram_pair(a[t], s) over LDS sources [0, ns) (ns a multiple of 64) */
 __device__ __forceinline__ void eval8_gram(const double4* __restrict__ src, const double* __restrict__ sw, int ns,
                                            const double4 (&a)[8], double acc[8], int lane) {
+#pragma unroll 2
   for (int s = lane; s < ns; s += 64) {
     const double4 p = src[s];
     const double w = sw[s];
