@@ -172,7 +172,7 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     shapes, _ = tuner.autotune(reps=3)
     del tuner
     ev = K.GpuEvaluator(mprob, precision=prec, graph=True, overlap=OVERLAP)
-    ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"], shapes["hadamard_form"])
+    ev.set_tuning(shapes["p2p_form"], shapes["hadamard_fb"], shapes["hadamard_form"], shapes["overlap"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
     t0 = time.perf_counter()
@@ -209,7 +209,7 @@ def tuned_nonuniform(K, steps, warmup, nv):
     t_tune = time.perf_counter() - t0
     del tuner
     ev = K.GpuEvaluator(prob, graph=True, overlap=OVERLAP)
-    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"])
+    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"], best["overlap"])
     for _ in range(warmup):
         ev.evaluate(copy=False)
     t0 = time.perf_counter()
@@ -220,7 +220,8 @@ def tuned_nonuniform(K, steps, warmup, nv):
     if nv > 0:
         err, _, _ = prob.validate_gpu(ev, prob.scatter(ev.evaluate(copy=True)[0]), nv)
     names = ("p2p_static", "p2p_persistent", "p2p_static64", "hadamard_valu_fb4", "hadamard_valu_fb8",
-             "hadamard_valu_fb16", "hadamard_mfma_fb4", "hadamard_mfma_fb8", "hadamard_mfma_fb16")
+             "hadamard_valu_fb16", "hadamard_mfma_fb4", "hadamard_mfma_fb8", "hadamard_mfma_fb16",
+             "layout_one_stream", "layout_overlap")
     out.update({"default": {"mpoints_s": n / (ms0 * 1e-3) / 1e6, "ms_per_step": ms0},
                 "tuned": {"mpoints_s": n / (ms1 * 1e-3) / 1e6, "ms_per_step": ms1, "choice": best,
                           "sampled_rel_err": err},

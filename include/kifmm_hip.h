@@ -43,8 +43,9 @@ enum {
                                   pair).  The list is also skipped by itself when it would
                                   exceed 16 pairs per target on average. */
   KIFMM_GPU_MIXED_INLINE = KIFMM_GPU_NO_CLOSE_LIST, /* the earlier name of the same flag */
-  KIFMM_GPU_OVERLAP = 4 /* run the U-list on a second stream of the handle (default priority)
-                           alongside the up pass and the V-list, joined before W, X and down;
+  KIFMM_GPU_OVERLAP = 4 /* start with the U-list on the handle's second stream (default
+                           priority) alongside the up pass and the V-list, joined before W, X
+                           and down (kifmm_gpu_tuning.overlap; the autotuner may change it);
                            the potentials are bit-identical to the one-stream order.  Phase
                            times then overlap: ulist is the U-list's own span. */
 };
@@ -61,7 +62,9 @@ typedef struct kifmm_gpu_tuning {
   int p2p_form;      /* KIFMM_P2P_*; the FP64 U-list only (the mixed path has one form) */
   int hadamard_fb;   /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
   int hadamard_form; /* KIFMM_HAD_*: V-list Hadamard on MFMA (default) or on the VALU */
-  int reserved[5];
+  int overlap;       /* stream layout: 0 one stream, 1 the U-list on the handle's second
+                        stream alongside up and V (initially 1 with KIFMM_GPU_OVERLAP) */
+  int reserved[4];
 } kifmm_gpu_tuning;
 enum {
   KIFMM_HAD_VALU = 0, /* a lane per parent, two waves per 64 parents */
@@ -147,20 +150,22 @@ int kifmm_gpu_plan(const kifmm_tree_view* tree, const kifmm_operators* ops, kifm
  * (n_trg doubles, gathered order). */
 int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long count);
 
-/* Set or read the launch shapes.  The U-list forms and the Hadamard block
- * sizes give bit-identical potentials; the two Hadamard forms sum in
- * different orders (rounding-level differences).  Setting drops a captured
- * graph (the next full evaluate captures again).  EINVAL for an unknown
- * value, EUNSUP for a U-list form other than STATIC on the mixed path. */
+/* Set or read the launch shapes and the stream layout.  The U-list forms,
+ * the Hadamard block sizes and the layouts give bit-identical potentials;
+ * the two Hadamard forms sum in different orders (rounding-level
+ * differences).  Setting drops a captured graph (the next full evaluate
+ * captures again).  EINVAL for an unknown value, EUNSUP for a U-list form
+ * other than STATIC on the mixed path. */
 int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t);
 int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
-/* Time every U-list form (FP64 path) on the U phase and every Hadamard
- * form and block size on the V phase of this handle's own input, reps
- * timed runs each after one warm-up, keep the fastest of each (minimum
- * over runs) and set it.  best (may be NULL) receives the choice; ms (may
- * be NULL, n_ms entries, up to 9) receives the times in the order STATIC,
- * PERSISTENT, STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16; -1 for a
- * candidate not timed. */
+/* Time every U-list form (FP64 path) on the U phase, every Hadamard form
+ * and block size on the V phase, then both stream layouts on whole (eager)
+ * evaluations of this handle's own input, reps timed runs each after one
+ * warm-up, keep the fastest of each (minimum over runs) and set it.  best
+ * (may be NULL) receives the choice; ms (may be NULL, n_ms entries, up to
+ * 11) receives the times in the order STATIC, PERSISTENT, STATIC64, VALU
+ * fb 4, 8, 16, MFMA fb 4, 8, 16, one stream, overlap; -1 for a candidate
+ * not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */

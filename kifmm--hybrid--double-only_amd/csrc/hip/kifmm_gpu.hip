@@ -1211,10 +1211,11 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
     }
   }
   g->overlap = opts ? (opts->flags & KIFMM_GPU_OVERLAP) != 0 : 0;
-  if (g->overlap && (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
-                     hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
-                     hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming) != hipSuccess ||
-                     hipEventCreate(&g->su0) != hipSuccess || hipEventCreate(&g->su1) != hipSuccess)) {
+  /* the second stream exists on every handle: the layout is a tuning choice */
+  if (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+      hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreate(&g->su0) != hipSuccess || hipEventCreate(&g->su1) != hipSuccess) {
     set_error("kifmm_gpu_create: second stream or its events could not be created");
     kifmm_gpu_destroy(g);
     return KIFMM_EDEVICE;
@@ -1360,20 +1361,23 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
   }
   if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_STATIC64 ||
       (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
-      (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA)) {
-    set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16 and "
-              "hadamard_form a KIFMM_HAD_* value");
+      (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA) ||
+      (t->overlap != 0 && t->overlap != 1)) {
+    set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16, "
+              "hadamard_form a KIFMM_HAD_* value and overlap 0 or 1");
     return KIFMM_EINVAL;
   }
   if (t->p2p_form != KIFMM_P2P_STATIC && g->precision != KIFMM_PREC_FP64) {
     set_error("kifmm_gpu_set_tuning: the U-list forms apply to the FP64 path");
     return KIFMM_EUNSUP;
   }
-  if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb && t->hadamard_form == g->had_form)
+  if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb && t->hadamard_form == g->had_form &&
+      t->overlap == g->overlap)
     return KIFMM_OK;
   g->p2p_form = t->p2p_form;
   g->had_fb = t->hadamard_fb;
   g->had_form = t->hadamard_form;
+  g->overlap = t->overlap;
   if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
   if (g->graph) (void)hipGraphDestroy(g->graph);
   g->gexec = nullptr;
@@ -1390,6 +1394,7 @@ int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t) {
   t->p2p_form = g->p2p_form;
   t->hadamard_fb = g->had_fb;
   t->hadamard_form = g->had_form;
+  t->overlap = g->overlap;
   return KIFMM_OK;
 }
 
@@ -1398,7 +1403,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[9] = {-1, -1, -1, -1, -1, -1, -1, -1, -1};
+  double t[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
   auto time_phase = [&](unsigned phase, double* out) -> int {
@@ -1434,10 +1439,28 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   }
   c.hadamard_form = best_v / 3;
   c.hadamard_fb = fbs[best_v % 3];
+  /* stream layout, on whole evaluations timed eagerly (a graph handle
+   * would otherwise capture here) */
+  const int use_graph = g->use_graph;
+  g->use_graph = 0;
+  int best_l = 0, rc = KIFMM_OK;
+  for (int l = 0; l < 2 && rc == KIFMM_OK; l++) {
+    c.overlap = l;
+    rc = kifmm_gpu_set_tuning(g, &c);
+    kifmm_phase_times pt{};
+    for (int r = 0; r <= reps && rc == KIFMM_OK; r++) {
+      rc = evaluate(g, KIFMM_PHASE_ALL, nullptr, &pt);
+      if (r == 1 || (r > 1 && pt.total < t[9 + l])) t[9 + l] = pt.total; /* run 0 warms up */
+    }
+    if (rc == KIFMM_OK && t[9 + l] < t[9 + best_l]) best_l = l;
+  }
+  g->use_graph = use_graph;
+  TRY(rc);
+  c.overlap = best_l;
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 9); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 11); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

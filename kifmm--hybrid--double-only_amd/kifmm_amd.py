@@ -73,7 +73,7 @@ class GpuOptions(C.Structure):
 
 class GpuTuning(C.Structure):
     _fields_ = [("p2p_form", C.c_int), ("hadamard_fb", C.c_int), ("hadamard_form", C.c_int),
-                ("reserved", C.c_int * 5)]
+                ("overlap", C.c_int), ("reserved", C.c_int * 4)]
 
 
 P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64 = 0, 1, 2
@@ -378,21 +378,26 @@ class GpuEvaluator:
     def tuning(self):
         t = GpuTuning()
         _check(lib.kifmm_gpu_get_tuning(self._h, C.byref(t)))
-        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form}
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
+                "overlap": t.overlap}
 
-    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8, hadamard_form=HAD_MFMA):
-        t = GpuTuning(p2p_form, hadamard_fb, hadamard_form)
+    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8, hadamard_form=HAD_MFMA, overlap=None):
+        """overlap=None keeps the handle's current stream layout."""
+        if overlap is None:
+            overlap = self.tuning()["overlap"]
+        t = GpuTuning(p2p_form, hadamard_fb, hadamard_form, int(overlap))
         _check(lib.kifmm_gpu_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, reps=3):
         """Time the launch shapes on this input, keep the fastest; returns
         (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, then the
         Hadamard on the VALU with 4, 8, 16 frequencies per block and on MFMA
-        with 4, 8, 16."""
+        with 4, 8, 16, then whole evaluations on one stream and overlapped."""
         t = GpuTuning()
-        ms = np.zeros(9)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 9))
-        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form}, ms
+        ms = np.zeros(11)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 11))
+        return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
+                "overlap": t.overlap}, ms
 
     def work_counts(self):
         w = WorkCounts()

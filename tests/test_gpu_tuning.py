@@ -4,9 +4,10 @@ knobs, fmm/util.c:118-163).
 
 Every U-list form (one block per item of <= 128 or <= 64 targets, or the
 persistent per-XCD work queue) and every Hadamard frequency block (4, 8,
-16) computes each output in the same order, so for one Hadamard form the
-potentials must be bit-identical to the default shapes, eagerly and through
-a recaptured HIP graph, with separate and coincident targets.  The MFMA
+16) computes each output in the same order, and so does either stream
+layout, so for one Hadamard form the potentials must be bit-identical to
+the default shapes, eagerly and through a recaptured HIP graph, with
+separate and coincident targets.  The MFMA
 Hadamard sums the same terms in another order: its potentials must meet the
 FP64 parity bar against the oracle themselves, agree with the VALU form's
 within that bar (the summation-order floor of the pseudo-inverse applies,
@@ -40,9 +41,10 @@ def test_every_shape_gives_identical_potentials(n, dist, q, np_, c):
     gr = K.GpuEvaluator(prob, graph=True)
     ref_mfma = None
     for form in (K.HAD_VALU, K.HAD_MFMA):
-        for f, fb in SHAPES:
-            ev.set_tuning(f, fb, form)
-            assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb, "hadamard_form": form}
+        for k, (f, fb) in enumerate(SHAPES):
+            ov = (k + form) % 2  # both stream layouts across the shapes
+            ev.set_tuning(f, fb, form, ov)
+            assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb, "hadamard_form": form, "overlap": ov}
             got, _ = ev.evaluate()
             if form == K.HAD_MFMA and ref_mfma is None:
                 ref_mfma = got
@@ -52,7 +54,7 @@ def test_every_shape_gives_identical_potentials(n, dist, q, np_, c):
             assert np.array_equal(got, want), (form, f, fb)
             if form == K.HAD_MFMA:
                 assert np.array_equal(got, default), ("default shapes", f, fb)
-            gr.set_tuning(f, fb, form)
+            gr.set_tuning(f, fb, form, ov)
             for _ in range(2):
                 got, _ = gr.evaluate()
                 assert np.array_equal(got, want), ("graph", form, f, fb)
@@ -82,11 +84,12 @@ def test_mixed_path_hadamard_blocks_and_forms():
 
 def test_invalid_tuning_is_rejected():
     ev = K.GpuEvaluator(K.Problem.create(5000, "uniform", 64, np_=4, seed=6))
-    for f, fb, form in ((3, 8, 0), (-1, 8, 0), (0, 5, 0), (0, 0, 0), (0, 8, 2), (0, 8, -1)):
+    for f, fb, form, ov in ((3, 8, 0, 0), (-1, 8, 0, 0), (0, 5, 0, 0), (0, 0, 0, 0), (0, 8, 2, 0), (0, 8, -1, 0),
+                            (0, 8, 1, 2), (0, 8, 1, -1)):
         with pytest.raises(K.KifmmError) as e:
-            ev.set_tuning(f, fb, form)
+            ev.set_tuning(f, fb, form, ov)
         assert e.value.code == -1
-    assert ev.tuning() == {"p2p_form": K.P2P_STATIC, "hadamard_fb": 8, "hadamard_form": K.HAD_MFMA}
+    assert ev.tuning() == {"p2p_form": K.P2P_STATIC, "hadamard_fb": 8, "hadamard_form": K.HAD_MFMA, "overlap": 0}
 
 
 @pytest.mark.parametrize("precision", [K.PREC_FP64, K.PREC_MIXED])
@@ -99,8 +102,9 @@ def test_autotune_keeps_the_fastest(precision):
     nforms = 3 if precision == K.PREC_FP64 else 1
     assert np.all(ms[:nforms] > 0) and np.all(ms[nforms:3] == -1) and np.all(ms[3:] > 0)
     assert best["p2p_form"] == int(np.argmin(ms[:nforms]))
-    v = int(np.argmin(ms[3:]))
+    v = int(np.argmin(ms[3:9]))
     assert (best["hadamard_form"], best["hadamard_fb"]) == (v // 3, (4, 8, 16)[v % 3])
+    assert best["overlap"] == int(np.argmin(ms[9:11]))
     assert ev.tuning() == best
     got, _ = ev.evaluate()
     assert rel(got, ref) < (O.PARITY_RTOL[4] if precision == K.PREC_FP64 else 1e-6)
