@@ -983,11 +983,18 @@ int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
 int enqueue(kifmm_gpu* g, unsigned mask) {
   hipEvent_t* ev = g->ev;
   const int lde = g->ld_e, lduc = g->ld_uc;
-  /* zero the accumulators (a fill kernel, so the sequence stays capturable) */
-  TRY(fill(g, g->pot, (size_t)g->n_trg));
-  TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
+  /* zero what this call would otherwise leave stale (a fill kernel, so the
+   * sequence stays capturable).  The U-list assigns pot for every target
+   * (each target leaf has items), the up pass assigns uden for every source
+   * node of depth >= 2 (the only ones read; the others are zero from create
+   * time and never written), and the down pass assigns dden for every
+   * target node of depth >= 2 (the only ones read), so each is cleared only
+   * when its phase is not in the mask; dchk is accumulated into (V, X, L2L)
+   * and always cleared. */
+  if (!(mask & KIFMM_PHASE_U)) TRY(fill(g, g->pot, (size_t)g->n_trg));
+  if (!(mask & KIFMM_PHASE_UP)) TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
   TRY(fill(g, g->dchk, (size_t)g->n_nodes * lde));
-  TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
+  if (!(mask & KIFMM_PHASE_DOWN)) TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
   /* KIFMM_GPU_OVERLAP: the U-list forks onto the side stream here and runs
    * alongside up and V.  It reads only points, lists and create-time data
    * and writes pot with '=' (the pot fill above is ordered before the

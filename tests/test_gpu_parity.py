@@ -162,3 +162,21 @@ def test_overlap_is_bitwise_identical(prec):
         for _ in range(3):
             assert np.array_equal(ev.evaluate()[0], a)
         assert np.array_equal(ev.evaluate_phases(K.PHASE_U)[0], u)
+
+
+def test_phase_subsets_after_a_full_evaluation_match_a_fresh_handle():
+    """Only the buffers a call's phases do not assign are cleared (pot by the
+    U-list, uden by the up pass, dden by the down pass; dchk always): after a
+    full evaluation every phase subset must give the potentials and
+    intermediates of a fresh handle, so no stale value leaks through."""
+    prob = K.Problem.create(40000, "ellipsoid", 32, np_=4, seed=12)
+    ev = K.GpuEvaluator(prob)
+    ev.evaluate()
+    for mask in (K.PHASE_U, K.PHASE_V | K.PHASE_DOWN, K.PHASE_UP | K.PHASE_V, K.PHASE_W | K.PHASE_X,
+                 K.PHASE_UP | K.PHASE_U | K.PHASE_DOWN, K.PHASE_ALL):
+        fresh = K.GpuEvaluator(prob)
+        want = fresh.evaluate_phases(mask)[0]
+        got = ev.evaluate_phases(mask)[0]
+        assert np.array_equal(got, want), mask
+        for name in ("uden", "dchk", "dden"):
+            assert np.array_equal(ev.intermediate(name), fresh.intermediate(name)), (mask, name)
