@@ -164,7 +164,7 @@ def mixed_vs_fp64(K, steps, warmup, nv):
     pts = prob.points()
     pts = tuple(pts[k] for k in ("sx", "sy", "sz", "den", "tx", "ty", "tz"))
     t0 = time.perf_counter()
-    q, prec, pred, how = CM.Tuner(CM.CostModel.load()).choose(pts, 4)
+    q, prec, pred, how = CM.Tuner(CM.CostModel.load()).choose(pts, 4, verify=2)
     t_choose = time.perf_counter() - t0
     mprob = K.Problem.from_points(*pts, pts_max=q, np_=4)
     mprob.operators()

@@ -210,7 +210,10 @@ def paths_for(np_: int):
 
 class Tuner:
     """Choose (q, precision) for one input: by the cost model when a model
-    is available, else by timing every candidate (exhaustive sweep)."""
+    is available, else by timing every candidate (exhaustive sweep).  With
+    verify = k > 1 the model ranks the candidates and the k best-ranked are
+    timed on the GPU, the fastest measured one kept: a guard against a
+    misranked near-tie at the cost of k tree builds and a few evaluations."""
 
     def __init__(self, model: CostModel | None = None, qs=Q_CHOICES, allow_mixed: bool = True):
         self.model = model
@@ -220,7 +223,7 @@ class Tuner:
     def _problem(self, pts, q, np_):
         return K.Problem.from_points(*pts, pts_max=q, np_=np_)
 
-    def choose(self, pts, np_: int):
+    def choose(self, pts, np_: int, verify: int = 1):
         """pts = (sx, sy, sz, den, tx, ty, tz) in the root box.  Returns
         (q, precision, predicted or measured ms, method)."""
         paths = paths_for(np_) if self.allow_mixed else (K.PREC_FP64,)
@@ -235,8 +238,17 @@ class Tuner:
                     rows.append(features(prob, q, pr, ops))
                     keys.append((q, pr))
             tot = self.model.predict_ms(np.stack(rows)).sum(1)
-            i = int(np.argmin(tot))
-            return keys[i][0], keys[i][1], float(tot[i]), "model"
+            if verify <= 1:
+                i = int(np.argmin(tot))
+                return keys[i][0], keys[i][1], float(tot[i]), "model"
+            for i in np.argsort(tot, kind="stable")[:verify]:
+                q, pr = keys[int(i)]
+                prob = self._problem(pts, q, np_)
+                prob.operators()
+                _, wall = phase_times(prob, pr)
+                if best is None or wall < best[2]:
+                    best = (q, pr, wall)
+            return best[0], best[1], best[2], "model+verify"
         for q in self.qs:
             prob = self._problem(pts, q, np_)
             prob.operators()

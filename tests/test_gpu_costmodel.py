@@ -66,3 +66,28 @@ def test_trained_model_gains_over_default_q():
         print(f"{dist} n={n} NP={np_}: model q={q} prec={prec} {t_model:.3f} ms; default q=64 FP64 {t_default:.3f} ms")
         rows.append(t_default / t_model)
     assert np.exp(np.mean(np.log(rows))) >= 1.0
+
+
+def test_verified_choice_is_one_of_the_best_ranked():
+    """Tuner(verify=2): the model ranks every (q, path), the two best-ranked
+    are timed, and the faster of them is returned (measured, not predicted)."""
+    model = CM.CostModel.load()
+    if model is None:
+        pytest.skip("no trained weights committed")
+    n, dist, np_, seed = 1 << 18, "uniform", 4, 9004
+    base = K.Problem.create(n, dist, 64, np_=np_, seed=seed)
+    pts = base.points()
+    pts = (pts["sx"], pts["sy"], pts["sz"], pts["den"], pts["tx"], pts["ty"], pts["tz"])
+    tuner = CM.Tuner(model)
+    rows, keys = [], []
+    ops = base.operators()
+    for q in tuner.qs:
+        prob = K.Problem.from_points(*pts, pts_max=q, np_=np_)
+        for pr in CM.paths_for(np_):
+            rows.append(CM.features(prob, q, pr, ops))
+            keys.append((q, pr))
+    order = np.argsort(model.predict_ms(np.stack(rows)).sum(1), kind="stable")
+    top2 = {keys[int(i)] for i in order[:2]}
+    q, prec, ms, how = tuner.choose(pts, np_, verify=2)
+    print(f"top-2 {sorted(top2)} -> q={q} prec={prec} {ms:.3f} ms", flush=True)
+    assert how == "model+verify" and (q, prec) in top2 and ms > 0
