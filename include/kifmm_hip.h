@@ -11,7 +11,8 @@
  *
  * All entry points return 0 on success or a negative KIFMM_E* code, never
  * exit.  A handle is not thread-safe; several handles may coexist.  Each
- * handle owns one HIP stream at default priority on its device.
+ * handle owns three HIP streams at default priority on its device: the
+ * main one, a second compute stream and a copy stream.
  */
 #ifndef KIFMM_HIP_H
 #define KIFMM_HIP_H
@@ -44,10 +45,12 @@ enum {
                                   exceed 16 pairs per target on average. */
   KIFMM_GPU_MIXED_INLINE = KIFMM_GPU_NO_CLOSE_LIST, /* the earlier name of the same flag */
   KIFMM_GPU_OVERLAP = 4 /* start with the U-list on the handle's second stream (default
-                           priority) alongside the up pass and the V-list, joined before W, X
-                           and down (kifmm_gpu_tuning.overlap; the autotuner may change it);
-                           the potentials are bit-identical to the one-stream order.  Phase
-                           times then overlap: ulist is the U-list's own span. */
+                           priority) alongside up, V, W, X and down, into a buffer of its own
+                           that one pass adds to the far field at the end
+                           (kifmm_gpu_tuning.overlap; the autotuner may change it).  The
+                           potentials are bit-identical to the default layout, but the copy
+                           back can no longer overlap the U-list.  Phase times then overlap:
+                           ulist is the U-list's own span. */
 };
 
 /* Launch shapes of a handle, the counterpart of the reference's hand-set
@@ -62,8 +65,9 @@ typedef struct kifmm_gpu_tuning {
   int p2p_form;      /* KIFMM_P2P_*; the FP64 U-list only (the mixed path has one form) */
   int hadamard_fb;   /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
   int hadamard_form; /* KIFMM_HAD_*: V-list Hadamard on MFMA (default) or on the VALU */
-  int overlap;       /* stream layout: 0 one stream, 1 the U-list on the handle's second
-                        stream alongside up and V (initially 1 with KIFMM_GPU_OVERLAP) */
+  int overlap;       /* stream layout: 0 the U-list last, in chunks whose copies back
+                        overlap the later chunks; 1 the U-list on the handle's second stream
+                        alongside everything else (initially 1 with KIFMM_GPU_OVERLAP) */
   int reserved[4];
 } kifmm_gpu_tuning;
 enum {
@@ -78,10 +82,11 @@ typedef struct kifmm_gpu_options {
   int reserved[5];
 } kifmm_gpu_options;
 
-/* Milliseconds, measured with HIP events on the handle's stream. */
+/* Milliseconds, measured with HIP events on the handle's stream.  The
+ * phases run in the order up, V, W, X, down, U-list. */
 typedef struct kifmm_phase_times {
   double up;        /* S2M + M2M                          */
-  double ulist;     /* U-list P2P                         */
+  double ulist;     /* U-list P2P (the last phase)        */
   double vlist;     /* sum of the four V sub-phases       */
   double v_fft_src; /* surface -> grid scatter + forward DFT */
   double v_trans;   /* translation spectra (316 vectors): formed once at
@@ -91,8 +96,12 @@ typedef struct kifmm_phase_times {
   double wlist;
   double xlist;
   double down;      /* L2L + DC2DE + L2T                  */
-  double copy;      /* device -> host potentials          */
-  double total;     /* first event to last event          */
+  double copy;      /* device -> host potentials: the part left after the last
+                       kernel (the copies of earlier U-list chunks overlap the later
+                       chunks), plus the host pass into a caller buffer that is not
+                       kifmm_gpu_output()'s; 0 in graph mode, whose copies are inside
+                       the graph (total covers them) */
+  double total;     /* first event to last event (+ that host pass) */
   double p2p_kernel;/* the U-list kernel alone            */
   double reserved[3];
 } kifmm_phase_times;
@@ -112,16 +121,48 @@ typedef struct kifmm_work_counts {
   double close_pairs;  /* U-list pairs closer than 1/8 of the target leaf's half-width,
                           listed at create time; -1 when not listed (KIFMM_GPU_NO_CLOSE_LIST
                           or too many) */
-  double reserved[4];
+  double copy_chunks;  /* U-list chunks whose copies back overlap the later chunks */
+  double reserved[3];
 } kifmm_work_counts;
 
 int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops,
                      const kifmm_gpu_options* opts, kifmm_gpu** out);
 /* Evaluate the potentials.  pot_leaf (host, n_trg doubles, gathered target
  * order) may be NULL: the result then stays in device memory and the copy
- * phase is skipped.  times may be NULL. */
+ * phase is skipped.  The copy goes to the handle's pinned host buffer,
+ * chunk by chunk while the U-list (the last phase) still computes; with
+ * pot_leaf == kifmm_gpu_output(g) that is all, any other pot_leaf gets one
+ * host pass from it (pipelined chunk by chunk in eager mode).  times may be
+ * NULL. */
 int kifmm_gpu_evaluate(kifmm_gpu* g, double* pot_leaf, kifmm_phase_times* times);
 int kifmm_gpu_destroy(kifmm_gpu* g);
+
+/* The handle's pinned host buffer of n_trg potentials (gathered target
+ * order), valid until kifmm_gpu_destroy; NULL if pinned memory could not be
+ * allocated.  Passing it to kifmm_gpu_evaluate copies the result at PCIe
+ * rate with no host pass: the counterpart of the host array that
+ * copy_trg_val__gpu fills (fmm/cuda.cu:2543-2565).  Each evaluation
+ * overwrites it. */
+double* kifmm_gpu_output(kifmm_gpu* g);
+
+/* New source densities (n_src doubles, gathered source order) for the
+ * handle's fixed geometry, for repeated evaluations with changing
+ * densities (in the reference a new AllNodes and preproc).  Everything
+ * create derives ahead of time (lists, close pairs, Gram records,
+ * translation spectra) depends on the points only; evaluations read the
+ * densities afresh.  Blocks until the upload is done. */
+int kifmm_gpu_set_densities(kifmm_gpu* g, const double* den_leaf);
+
+/* Wall milliseconds of the handle's kifmm_gpu_create: [0] the host plan
+ * (work lists and their bounds checks), [1] the upload, [2] the close-pair
+ * search, [3] the other device set-up (translation spectra, Gram records),
+ * [4] the whole call.  n entries (up to 5) are written. */
+int kifmm_gpu_create_times(const kifmm_gpu* g, double* ms, int n);
+
+/* The close-pair list for tests: n_ptr <= n_trg + 1 offsets and n_idx
+ * source indices (gathered order), per target in U-list order.  Returns
+ * the number of listed pairs, or KIFMM_EUNSUP when the handle has none. */
+long kifmm_gpu_close_list(const kifmm_gpu* g, int* ptr, long n_ptr, int* idx, long n_idx);
 
 /* Phase selection, the counterpart of the reference's separately callable
  * phase functions up_calc__gpu, ulist_calc__gpu, vlist_calc__gpu,

@@ -7,7 +7,7 @@
  *                              + get_input()   (fmm/input.cc:29-84,203-460,
  *                              fmm/driver.cc:244-299, fmm/evaluate.cc:57-96)
  *   kifmm_problem_operators .. compute_UC2UE_mat / UE2UC / DC2DE / DE2DC
- *                              (fmm/trans.cc:203-326; pinv fmm/vecmatop.cc:34-78)
+ *                              (fmm/trans.cc:143-264; pinv fmm/vecmatop.cc:34-78)
  *   kifmm_problem_scatter .... copy_trg_val's scatter to input order
  *                              (fmm/omp_sse_block.cc:873-887, set_value
  *                              fmm/evaluate.cc:192-202)
@@ -69,7 +69,7 @@ typedef struct kifmm_tree_view {
   int coincident;         /* 1 when targets are the sources (r == 0 pairs occur) */
 } kifmm_tree_view;
 
-/* Translation operators for the unit box (fmm/trans.cc:203-326), FP64,
+/* Translation operators for the unit box (fmm/trans.cc:143-264), FP64,
  * column-major.  n_e = np^3 - (np-2)^3, n_uc = (np+2)^3 - np^3. */
 typedef struct kifmm_operators {
   int np, n_e, n_uc;

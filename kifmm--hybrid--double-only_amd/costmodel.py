@@ -219,6 +219,7 @@ class Tuner:
         self.model = model
         self.qs = tuple(qs)
         self.allow_mixed = allow_mixed
+        self.model_pick = None  # (q, precision, predicted ms) of the last model ranking
 
     def _problem(self, pts, q, np_):
         return K.Problem.from_points(*pts, pts_max=q, np_=np_)
@@ -238,6 +239,8 @@ class Tuner:
                     rows.append(features(prob, q, pr, ops))
                     keys.append((q, pr))
             tot = self.model.predict_ms(np.stack(rows)).sum(1)
+            i0 = int(np.argmin(tot))
+            self.model_pick = (keys[i0][0], keys[i0][1], float(tot[i0]))  # the network's own choice
             if verify <= 1:
                 i = int(np.argmin(tot))
                 return keys[i][0], keys[i][1], float(tot[i]), "model"

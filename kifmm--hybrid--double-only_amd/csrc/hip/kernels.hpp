@@ -168,7 +168,7 @@ __device__ __forceinline__ int walk_tile(RangeWalk& w, const int2* __restrict__ 
 }
 
 /* FP64 U-list on the Gram form (used when the close pairs are listed, see
- * k_close_pairs).  Coordinates are centred on the target leaf and scaled by
+ * k_close_near).  Coordinates are centred on the target leaf and scaled by
  * sc = GRAM_SC / h (a power of two: exact), so a pair is close, r <
  * CLOSE_FRAC h, exactly when its scaled distance r' is below 2.  Per pair
  *     r'^2 = (|t|^2 + |s|^2) + (-2t) . s              (1 add, 3 FMA)
@@ -229,7 +229,7 @@ template <bool MASK, int G, bool GRAM>
 __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, const int* __restrict__ rng_ptr,
                                          const int2* __restrict__ rng, const double4* __restrict__ src4,
                                          const double4* __restrict__ trg4, double* __restrict__ pot,
-                                         const GramArgs& ga) {
+                                         const GramArgs& ga, int accum) {
   const int tid = threadIdx.x, lane = tid & 63;
   /* groups g = wave' + 4k with wave' the wave index rotated by the item:
    * when the group count is not a multiple of 4 the short wave moves from
@@ -354,8 +354,8 @@ __device__ __forceinline__ void p2p_item(double4* tile, const int4 it, int rot, 
       const double sum = wave_reduce8(acc[k], lane);
       const int t = 8 * grp + (lane >> 3);
       if ((lane & 7) == 0 && t < it.z) {
-        if constexpr (GRAM) pot[it.y + t] = (HALF_OOFP_R * sc) * (sum + ga.sc_sum[t]);
-        else pot[it.y + t] = HALF_OOFP_R * sum;
+        const double v = GRAM ? (HALF_OOFP_R * sc) * (sum + ga.sc_sum[t]) : HALF_OOFP_R * sum;
+        pot[it.y + t] = accum ? v + pot[it.y + t] : v;
       }
     }
   }
@@ -369,7 +369,8 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
                                                    const double4* __restrict__ trg4, double* __restrict__ pot,
                                                    const double4* __restrict__ geo,
                                                    const double4* __restrict__ trg_gram,
-                                                   const int* __restrict__ cptr, const int* __restrict__ cidx) {
+                                                   const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                                   int accum) {
   __shared__ double4 tile[P2P_TILE];
   __shared__ double sw[GRAM ? P2P_TILE : 1];
   __shared__ double sc_sum[GRAM ? 32 * G : 1];
@@ -378,7 +379,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist(const int4* __restrict__ item
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
   const GramArgs ga{sw, sc_sum, geo, trg_gram, cptr, cidx};
-  p2p_item<MASK, G, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
+  p2p_item<MASK, G, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga, accum);
 }
 
 /* The same with items of at most 64 targets (G = 2 groups per wave): the
@@ -392,14 +393,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6, 8))) voi
     const int4* __restrict__ items, int n_items, int per_xcd, const int* __restrict__ rng_ptr,
     const int2* __restrict__ rng, const double4* __restrict__ src4, const double4* __restrict__ trg4,
     double* __restrict__ pot, const double4* __restrict__ geo, const double4* __restrict__ trg_gram,
-    const int* __restrict__ cptr, const int* __restrict__ cidx) {
+    const int* __restrict__ cptr, const int* __restrict__ cidx, int accum) {
   __shared__ double4 tile[P2P_TILE];
   __shared__ double sw[GRAM ? P2P_TILE : 1];
   __shared__ double sc_sum[GRAM ? P2P_MAXT : 1];
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
   const GramArgs ga{sw, sc_sum, geo, trg_gram, cptr, cidx};
-  p2p_item<MASK, P2P_G / 2, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga);
+  p2p_item<MASK, P2P_G / 2, GRAM>(tile, items[item], item, rng_ptr, rng, src4, trg4, pot, ga, accum);
 }
 
 /* Persistent form over the same items (<= 32 G targets): a fixed grid
@@ -416,7 +417,8 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restric
                                                            int* __restrict__ counters,
                                                            const double4* __restrict__ geo,
                                                            const double4* __restrict__ trg_gram,
-                                                           const int* __restrict__ cptr, const int* __restrict__ cidx) {
+                                                           const int* __restrict__ cptr, const int* __restrict__ cidx,
+                                                           int accum) {
   __shared__ double4 tile[P2P_TILE];
   __shared__ double sw[GRAM ? P2P_TILE : 1];
   __shared__ double sc_sum[GRAM ? 32 * G : 1];
@@ -430,7 +432,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_persist(const int4* __restric
     const int k = draw;
     __syncthreads(); /* every thread has read draw before the next draw overwrites it */
     if (k >= n) break;
-    p2p_item<MASK, G, GRAM>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot, ga);
+    p2p_item<MASK, G, GRAM>(tile, items[lo + k], lo + k, rng_ptr, rng, src4, trg4, pot, ga, accum);
   }
 }
 
@@ -533,7 +535,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
                                                          const double4* __restrict__ geo,
                                                          const double4* __restrict__ src4,
                                                          const double4* __restrict__ trg4,
-                                                         double* __restrict__ pot) {
+                                                         double* __restrict__ pot, int accum) {
   __shared__ float4 sB[P2P_TILE];
   __shared__ float sW[P2P_TILE];
   __shared__ double4 sS[P2P_TILE];
@@ -614,7 +616,10 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
         v += __shfl_xor(v, 2, 64);
         v += __shfl_xor(v, 1, 64);
         const int t = 16 * grp + 4 * (lane >> 4) + e;
-        if ((lane & 15) == 0 && t < it.z) pot[it.y + t] = OOFP_R * ih * v;
+        if ((lane & 15) == 0 && t < it.z) {
+          const double u = OOFP_R * ih * v;
+          pot[it.y + t] = accum ? u + pot[it.y + t] : u;
+        }
       }
     }
   }
@@ -637,7 +642,7 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
  *
  * Every pair then adds w * min(1, 1 / r') in FP32, the min and a sign guard
  * being the free clamp and |x| modifiers of v_rsq_f32: no per-pair test.
- * The pairs with r' < 1 in FP64 were found once per plan (k_close_pairs:
+ * The pairs with r' < 1 in FP64 were found once per plan (k_close_near:
  * per target, a CSR list of source indices); when a target's sum is
  * written, each adds w (1 / r' - 1) in FP64 (w (0 - 1) for r = 0 with
  * MASK), which replaces the clamped w by the FP64 term.  Pairs near r' = 1
@@ -741,7 +746,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
                                                              const double4* __restrict__ trg4,
                                                              const int* __restrict__ cptr,
                                                              const int* __restrict__ cidx,
-                                                             double* __restrict__ pot) {
+                                                             double* __restrict__ pot, int accum) {
   __shared__ v8bf sB[4][P2P_TILE];
   __shared__ float sW[P2P_TILE];
   __shared__ double4 sT[P2P_MAXT5];
@@ -858,74 +863,193 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         v += __shfl_xor(v, 2, 64);
         v += __shfl_xor(v, 1, 64);
         const int t = 16 * grp + 4 * (lane >> 4) + e;
-        if ((lane & 15) == 0 && t < it.z) pot[it.y + t] = scale * (v + sC[t]);
+        if ((lane & 15) == 0 && t < it.z) {
+          const double u = scale * (v + sC[t]);
+          pot[it.y + t] = accum ? u + pot[it.y + t] : u;
+        }
       }
     }
   }
 }
 
 /* The close pairs of the U-list (geometry only, found once per plan): for
- * each target of an item, the sources of its U-list ranges at FP64 scaled
- * distance r' < 1 (r < MIX_CLOSE h), in range order.  FILL = false writes
- * the count per target; FILL = true writes the source indices from the
- * exclusive offsets cptr.  Items in the U-list kernels' XCD-contiguous
- * order (8 per_xcd blocks), so neighbouring leaves share an L2. */
+ * each target of an item, the sources of its leaf's U-list boxes at FP64
+ * scaled distance r' < 1 (r < MIX_CLOSE h, h the target leaf's half-width),
+ * in U-list order and, inside a box, in source order: the order of the
+ * U-list kernels' range walk.  One wave per item (<= P2P_MAXT5 targets).
+ * Only boxes that some target lies within r' < 1 (+ margin) of are read:
+ * for a target inside its own leaf that is the leaf itself, plus the face
+ * neighbours of the targets within h/8 of a face and rarely an edge or
+ * corner neighbour, so a uniform leaf of q points costs about 1.3 q^2 pair
+ * tests instead of the 27 q^2 of the whole U-list.  Each candidate pair is
+ * classified by exactly the expression of the all-pairs search it replaces
+ * (scaled differences, then fma(dz, dz, fma(dy, dy, dx * dx)) < 1), so the
+ * list is the same.  FILL = false writes the count per target; FILL = true
+ * writes the source indices from the exclusive offsets cptr.  Items in the
+ * U-list kernels' XCD-contiguous order, so neighbouring leaves share an L2. */
+constexpr int CLOSE_TC = (P2P_MAXT5 + 63) / 64; /* target chunks of 64 per item */
 template <bool FILL>
-__global__ __launch_bounds__(128) void k_close_pairs(const int4* __restrict__ items, int n_items, int per_xcd,
-                                                     const int* __restrict__ rng_ptr,
-                                                     const int2* __restrict__ rng, const double4* __restrict__ geo,
-                                                     const double4* __restrict__ src4,
-                                                     const double4* __restrict__ trg4, int* __restrict__ cnt,
-                                                     const int* __restrict__ cptr, int* __restrict__ cidx) {
-  __shared__ double4 tile[P2P_TILE];
+__global__ __launch_bounds__(64) void k_close_near(const int4* __restrict__ items, int n_items, int per_xcd,
+                                                   const int* __restrict__ u_ptr, const int* __restrict__ u_idx,
+                                                   const int* __restrict__ sbeg, const int* __restrict__ snum,
+                                                   const double4* __restrict__ geo,
+                                                   const double4* __restrict__ src4,
+                                                   const double4* __restrict__ trg4, int* __restrict__ cnt,
+                                                   const int* __restrict__ cptr, int* __restrict__ cidx) {
+  __shared__ double4 sT[64 * CLOSE_TC];
+  __shared__ int run[64 * CLOSE_TC];
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
-  if (item >= n_items) return; /* whole block, before any barrier */
+  if (item >= n_items) return; /* whole block (one wave) */
   const int4 it = items[item];
+  const int lane = threadIdx.x;
   const double4 g = geo[it.w];
-  const double sc = (1.0 / MIX_CLOSE) / g.w;
-  const int k = threadIdx.x;
-  const bool live = k < it.z;
-  double tx = 0.0, ty = 0.0, tz = 0.0;
-  if (live) {
-    const double4 q = trg4[it.y + k];
-    tx = (q.x - g.x) * sc; ty = (q.y - g.y) * sc; tz = (q.z - g.z) * sc;
-  }
-  int n = 0, pos = 0, end = 0;
-  if (FILL && live) {
-    pos = cptr[it.y + k];
-    end = cptr[it.y + k + 1];
-  }
-  int ri = rng_ptr[it.x], ro = 0;
-  const int rlast = rng_ptr[it.x + 1];
-  while (ri < rlast) {
-    __syncthreads();
-    int fill = 0;
-    while (fill < P2P_TILE && ri < rlast) {
-      const int2 r = rng[ri];
-      const int take = min(r.y - ro, P2P_TILE - fill);
-      for (int m = k; m < take; m += 128) {
-        const double4 q = src4[r.x + ro + m];
-        tile[fill + m] = make_double4((q.x - g.x) * sc, (q.y - g.y) * sc, (q.z - g.z) * sc, (double)(r.x + ro + m));
-      }
-      fill += take;
-      ro += take;
-      if (ro == r.y) {
-        ri++;
-        ro = 0;
-      }
+  const double sc = (1.0 / MIX_CLOSE) / g.w; /* a power of two: exact */
+#pragma unroll
+  for (int c = 0; c < CLOSE_TC; c++) {
+    const int k = lane + 64 * c;
+    double4 t = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (k < it.z) {
+      const double4 q = trg4[it.y + k];
+      t = make_double4((q.x - g.x) * sc, (q.y - g.y) * sc, (q.z - g.z) * sc, 0.0);
     }
-    __syncthreads();
-    if (live)
-      for (int m = 0; m < fill; m++) {
-        const double4 sp = tile[m];
-        const double dx = tx - sp.x, dy = ty - sp.y, dz = tz - sp.z;
-        if (__builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) < 1.0) {
-          if (FILL && pos + n < end) cidx[pos + n] = (int)sp.w; /* the count pass sized it */
-          n++;
+    sT[k] = t;
+    run[k] = (FILL && k < it.z) ? cptr[it.y + k] : 0;
+  }
+  __syncthreads();
+  for (int u = u_ptr[it.w]; u < u_ptr[it.w + 1]; u++) {
+    const int s_node = u_idx[u];
+    const int ns = snum[s_node];
+    if (ns <= 0) continue;
+    const int sb = sbeg[s_node];
+    const double4 gs = geo[s_node];
+    /* the box of s_node in the item's scaled coordinates */
+    const double cx = (gs.x - g.x) * sc, cy = (gs.y - g.y) * sc, cz = (gs.z - g.z) * sc, hw = gs.w * sc;
+    unsigned long long near[CLOSE_TC];
+    unsigned long long any = 0;
+#pragma unroll
+    for (int c = 0; c < CLOSE_TC; c++) {
+      const int k = lane + 64 * c;
+      const double4 t = sT[k];
+      const double ex = __builtin_fmax(__builtin_fabs(t.x - cx) - hw, 0.0);
+      const double ey = __builtin_fmax(__builtin_fabs(t.y - cy) - hw, 0.0);
+      const double ez = __builtin_fmax(__builtin_fabs(t.z - cz) - hw, 0.0);
+      /* a point of the box lies at least this far from the target; the
+       * margin covers the rounding of the scaled coordinates */
+      near[c] = __ballot(k < it.z && ex * ex + ey * ey + ez * ez < 1.0 + 1e-6);
+      any |= near[c];
+    }
+    if (!any) continue;
+    for (int base = 0; base < ns; base += 64) {
+      const bool sl = base + lane < ns;
+      const int si = sb + base + lane;
+      double sx = 1e30, sy = 1e30, sz = 1e30;
+      if (sl) {
+        const double4 q = src4[si];
+        sx = (q.x - g.x) * sc; sy = (q.y - g.y) * sc; sz = (q.z - g.z) * sc;
+      }
+#pragma unroll
+      for (int c = 0; c < CLOSE_TC; c++) {
+        unsigned long long m = near[c];
+        while (m) {
+          const int b = __builtin_ctzll(m);
+          m &= m - 1;
+          const int k = 64 * c + b;
+          const double4 t = sT[k];
+          const double dx = t.x - sx, dy = t.y - sy, dz = t.z - sz;
+          const bool close = sl && __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx)) < 1.0;
+          const unsigned long long cm = __ballot(close);
+          if (cm) {
+            const int r = run[k];
+            if (FILL && close) {
+              const int pos = r + (int)__builtin_amdgcn_mbcnt_hi((unsigned)(cm >> 32),
+                                                                 __builtin_amdgcn_mbcnt_lo((unsigned)cm, 0u));
+              if (pos < cptr[it.y + k + 1]) cidx[pos] = si; /* the count pass sized it */
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (lane == 0) run[k] = r + __popcll(cm);
+            __builtin_amdgcn_wave_barrier();
+          }
         }
       }
+    }
   }
-  if (!FILL && live) cnt[it.y + k] = n;
+  __syncthreads();
+  if (!FILL)
+#pragma unroll
+    for (int c = 0; c < CLOSE_TC; c++) {
+      const int k = lane + 64 * c;
+      if (k < it.z) cnt[it.y + k] = run[k];
+    }
+}
+
+/* Exclusive prefix sums of n counts into out[0..n] (out[n] = the total), in
+ * a fixed order: k_scan_blocks sums SCAN_PER consecutive counts per block
+ * (int64), k_scan_tops scans those block sums in one block, k_scan_out writes
+ * each block's exclusive sums from its offset.  n <= SCAN_PER * SCAN_PER. */
+constexpr int SCAN_T = 1024, SCAN_V = 16, SCAN_PER = SCAN_T * SCAN_V;
+__device__ __forceinline__ long long block_exclusive(long long v, long long* sh, long long* total) {
+  /* inclusive scan over the block by Hillis-Steele in LDS, then shift */
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+  for (int off = 1; off < SCAN_T; off <<= 1) {
+    const long long u = t >= off ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += u;
+    __syncthreads();
+  }
+  const long long inc = sh[t];
+  *total = sh[SCAN_T - 1];
+  __syncthreads();
+  return inc - v;
+}
+static __global__ __launch_bounds__(SCAN_T) void k_scan_blocks(const int* __restrict__ in, long long n,
+                                                               long long* __restrict__ bsum) {
+  __shared__ long long sh[SCAN_T];
+  const long long b0 = (long long)blockIdx.x * SCAN_PER + (long long)threadIdx.x * SCAN_V;
+  long long s = 0;
+  for (int k = 0; k < SCAN_V; k++)
+    if (b0 + k < n) s += in[b0 + k];
+  long long tot;
+  block_exclusive(s, sh, &tot);
+  if (threadIdx.x == 0) bsum[blockIdx.x] = tot;
+}
+static __global__ __launch_bounds__(SCAN_T) void k_scan_tops(long long* __restrict__ bsum, int nb,
+                                                             long long* __restrict__ total) {
+  __shared__ long long sh[SCAN_T];
+  const int b0 = threadIdx.x * SCAN_V;
+  long long v[SCAN_V], s = 0;
+  for (int k = 0; k < SCAN_V; k++) {
+    v[k] = b0 + k < nb ? bsum[b0 + k] : 0;
+    s += v[k];
+  }
+  long long tot;
+  long long run = block_exclusive(s, sh, &tot);
+  for (int k = 0; k < SCAN_V; k++)
+    if (b0 + k < nb) {
+      bsum[b0 + k] = run;
+      run += v[k];
+    }
+  if (threadIdx.x == 0) *total = tot;
+}
+/* out is written as int: the caller checks the total against its cap first */
+static __global__ __launch_bounds__(SCAN_T) void k_scan_out(const int* __restrict__ in, long long n,
+                                                            const long long* __restrict__ boff,
+                                                            const long long* __restrict__ total,
+                                                            int* __restrict__ out) {
+  __shared__ long long sh[SCAN_T];
+  const long long b0 = (long long)blockIdx.x * SCAN_PER + (long long)threadIdx.x * SCAN_V;
+  long long s = 0;
+  for (int k = 0; k < SCAN_V; k++)
+    if (b0 + k < n) s += in[b0 + k];
+  long long tot;
+  long long run = boff[blockIdx.x] + block_exclusive(s, sh, &tot);
+  for (int k = 0; k < SCAN_V; k++)
+    if (b0 + k < n) {
+      out[b0 + k] = (int)run;
+      run += in[b0 + k];
+    }
+  if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = (int)*total;
 }
 
 /* ------------------------------------------------------------------ S2M */
@@ -1264,7 +1388,7 @@ inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, i
 
 /* Grid geometry for accuracy NP: N = 2 NP points per axis, H = N/2 + 1
  * half-spectrum length along x, F = N*N*H complex frequencies, the surface
- * sits at grid offset NP/2 (fmm/trans.cc:395-417). */
+ * sits at grid offset NP/2 (fmm/trans.cc:335-355,394-416). */
 /* Storage of the V-list spectra: double2 on the FP64 path, float2 on the
  * mixed path (FP32 complex M2L).  R is the arithmetic type of the Hadamard
  * multiply-accumulate; the DFTs compute in FP64 and round on store. */
@@ -2034,6 +2158,22 @@ static __global__ void k_fill(double* __restrict__ p, size_t n, double v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) p[i] = v;
+}
+
+/* pot = u + pot over [0, n): the U-list sums (u) written on the second
+ * stream joined with the far field (W, X, L2T) accumulated in pot, in the
+ * same operand order as the U-list kernels' own accumulating write */
+static __global__ void k_join_pot(double* __restrict__ pot, const double* __restrict__ u, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) pot[i] = u[i] + pot[i];
+}
+
+/* new densities (leaf order) into the w field of the source records */
+static __global__ void k_set_w(double4* __restrict__ src4, const double* __restrict__ w, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) src4[i].w = w[i];
 }
 
 }  // namespace kifmm_dev

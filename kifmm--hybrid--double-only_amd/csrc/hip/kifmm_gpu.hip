@@ -24,6 +24,7 @@
 
 #include <algorithm>
 #include <array>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -82,6 +83,7 @@ struct kifmm_gpu {
   int *cube2surf = nullptr, *surf2cube = nullptr;
   int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *depth = nullptr;
   int *widx = nullptr, *xidx = nullptr;
+  int *u_ptr = nullptr, *u_idx = nullptr; /* U-list CSR (node ids), for the close-pair search */
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
@@ -117,13 +119,28 @@ struct kifmm_gpu {
   double* ws = nullptr;
   size_t ws_len = 0;
   kifmm_work_counts counts{};
+  /* wall ms of kifmm_gpu_create: host plan, upload, close-pair search,
+   * other device set-up (translation spectra, Gram records), whole call */
+  double create_ms[5] = {0, 0, 0, 0, 0};
   /* pinned host staging of the potentials (device -> host copy at PCIe rate) */
   double* pinned = nullptr;
   /* HIP graph of the whole launch sequence (KIFMM_GPU_GRAPH) */
   int use_graph = 0;
   bool capturing = false;
-  hipGraph_t graph = nullptr;
-  hipGraphExec_t gexec = nullptr;
+  hipGraph_t graph[2] = {nullptr, nullptr}; /* [1]: with the copy-out */
+  hipGraphExec_t gexec[2] = {nullptr, nullptr};
+  /* copy-out: target cuts of the U-list chunks (leaf starts, 0 .. n_trg),
+   * the first item of each chunk per item list, the copy stream (default
+   * priority) and the events that order chunks, copies and joins */
+  std::vector<int> cuts;
+  std::vector<int> chunk_beg[3];
+  hipStream_t cstream = nullptr;
+  std::vector<hipEvent_t> ch_ev, cd_ev;
+  hipEvent_t far_ev = nullptr, sdone_ev = nullptr, cdone_ev = nullptr;
+  /* the U-list's own sums when it runs on the second stream (overlap layout) */
+  double* potU = nullptr;
+  /* staging of kifmm_gpu_set_densities */
+  double* den_stage = nullptr;
 };
 
 namespace {
@@ -168,7 +185,7 @@ int dev_zero(kifmm_gpu* g, T** dst, size_t n) {
     if (r_ != KIFMM_OK) return r_; \
   } while (0)
 
-/* boundary nodes of an np^3 grid on [-R,R]^3, x slowest (fmm/trans.cc:91-120):
+/* boundary nodes of an np^3 grid on [-R,R]^3, x slowest (fmm/trans.cc:31-62):
  * rows x, y, z, then |p|^2 (row 3, for the Gram-form L2T) */
 void host_surface(int np, double R, std::vector<double>& xyz, int ld) {
   const double step = 2.0 / (np - 1);
@@ -245,6 +262,11 @@ struct Plan {
   int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
   int N = 0, H = 0, F = 0, nblk = 0, max_depth = 0;
   std::vector<int4> p2p_items, p2p_items64, p2p_items5;
+  /* copy-out chunks: target cut points (leaf starts, 0 .. n_trg) and, per
+   * item list (<= P2P_MAXT, <= P2P_MAXT / 2, <= P2P_MAXT5), the first item
+   * of each chunk */
+  std::vector<int> cuts;
+  std::vector<int> chunk_beg[3];
   std::vector<int> rng_ptr;
   std::vector<int2> rng;
   std::vector<int4> s2m_items;
@@ -327,6 +349,52 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
         return fail("U-list work item out of range");
   for (const int2& r : P.rng)
     if (r.x < 0 || r.y < 1 || (long)r.x + r.y > t->n_src) return fail("U-list source range out of range");
+  /* Copy-out chunks.  The U-list runs last, chunk by chunk, and each
+   * chunk's potentials go to the host while the next chunks compute.  A
+   * chunk is a run of whole leaves whose targets are one contiguous range:
+   * the item lists cover [0, n_trg) in item order with no gap (checked), and
+   * cuts fall at leaf starts, where every list has an item boundary.  The
+   * chunks shrink towards the end so that the copy left after the last
+   * chunk is short.  Small inputs are one chunk. */
+  {
+    const std::vector<int4>* lists[3] = {&P.p2p_items, &P.p2p_items64, &P.p2p_items5};
+    bool tiled = t->n_trg > 0;
+    for (const std::vector<int4>* L : lists) {
+      long next = 0;
+      for (const int4& it : *L) {
+        if (it.y != next) tiled = false;
+        next = (long)it.y + it.z;
+      }
+      if (next != t->n_trg) tiled = false;
+    }
+    P.cuts.assign(1, 0);
+    if (tiled && t->n_trg >= (1 << 18)) {
+      static const double frac[] = {0.30, 0.55, 0.75, 0.88, 0.95};
+      for (double f : frac) {
+        const long want = (long)(f * t->n_trg);
+        /* the first leaf start at or after want */
+        int cut = -1;
+        for (const int4& it : P.p2p_items5)
+          if (it.y >= want && it.y == t->trg_beg[it.w]) {
+            cut = it.y;
+            break;
+          }
+        if (cut > P.cuts.back() && cut < t->n_trg) P.cuts.push_back(cut);
+      }
+    }
+    P.cuts.push_back(t->n_trg);
+    for (int l = 0; l < 3; l++) {
+      const std::vector<int4>& L = *lists[l];
+      P.chunk_beg[l].clear();
+      size_t k = 0;
+      for (size_t c = 0; c + 1 < P.cuts.size(); c++) {
+        while (k < L.size() && L[k].y < P.cuts[c]) k++;
+        if (k < L.size() && L[k].y != P.cuts[c] && c > 0) return fail("copy-out chunk does not start an item");
+        P.chunk_beg[l].push_back((int)k);
+      }
+      P.chunk_beg[l].push_back((int)L.size());
+    }
+  }
 
   /* S2M + UC2UE over source leaves */
   for (int i = 0; i < nn; i++) {
@@ -570,9 +638,7 @@ int upload_gemm(kifmm_gpu* g, GemmList& L, const HostGemm& h) {
   return KIFMM_OK;
 }
 
-int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
-  Plan P;
-  TRY(make_plan(t, ops, P));
+int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, const Plan& P) {
   const int np = P.np, nn = t->n_nodes, ne = P.n_e, nuc = P.n_uc, lde = P.ld_e, lduc = P.ld_uc;
   g->np = np; g->n_e = ne; g->n_uc = nuc; g->ld_e = lde; g->ld_uc = lduc;
   g->lda_e = P.lda_e; g->lda_uc = P.lda_uc; g->lda_l2l = P.lda_l2l; g->ld_surf = P.ld_surf;
@@ -608,6 +674,8 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
     TRY(dev_upload(g, &g->depth, t->depth, nn));
     TRY(dev_upload(g, &g->widx, t->w_idx, t->w_ptr[nn]));
     TRY(dev_upload(g, &g->xidx, t->x_idx, t->x_ptr[nn]));
+    TRY(dev_upload(g, &g->u_ptr, t->u_ptr, (size_t)nn + 1));
+    TRY(dev_upload(g, &g->u_idx, t->u_idx, t->u_ptr[nn]));
     TRY(dev_zero(g, &g->uden, (size_t)nn * lde));
     TRY(dev_zero(g, &g->dchk, (size_t)nn * lde));
     TRY(dev_zero(g, &g->dden, (size_t)nn * lde));
@@ -635,7 +703,7 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
           d[(size_t)k * g->lda_l2l + o * lde + i] = ops->de2dc[(size_t)o * ne * ne + (size_t)k * ne + i];
     TRY(dev_upload(g, &g->DE2DC, d.data(), d.size()));
   }
-  /* unit surfaces (fmm/trans.cc:73-86) and DFT tables */
+  /* unit surfaces (fmm/trans.cc:13-26) and DFT tables */
   {
     const int ls = g->ld_surf;
     std::vector<double> s(4 * (size_t)ls, 0.0);
@@ -679,7 +747,11 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
   TRY(dev_upload(g, &g->p2p_items64, P.p2p_items64.data(), P.p2p_items64.size()));
   g->n_p2p5 = (int)P.p2p_items5.size();
   TRY(dev_upload(g, &g->p2p_items5, P.p2p_items5.data(), P.p2p_items5.size()));
-  TRY(dev_zero(g, &g->p2p_counters, 8));
+  TRY(dev_zero(g, &g->p2p_counters, 8 * std::max<size_t>(P.cuts.size(), 1)));
+  g->cuts = P.cuts;
+  for (int l = 0; l < 3; l++) g->chunk_beg[l] = P.chunk_beg[l];
+  TRY(dev_zero(g, &g->potU, (size_t)t->n_trg));
+  TRY(dev_zero(g, &g->den_stage, (size_t)t->n_src));
   {
     int ncu = 0, per_cu = 0;
     HIP_TRY(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, g->dev));
@@ -739,50 +811,63 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops) {
 }
 
 /* Mixed path, once per evaluator: leaves per block for the mixed S2M and
- * L2T, and the sources in FP32 leaf-local units
- * (positions and densities are fixed for the handle's lifetime) for
- * k_s2m_mixed. */
+ * L2T.  (The FP32 leaf-local source records k_s2m_mixed reads are written
+ * by k_src_local32 at the start of every up pass, so that they follow the
+ * current densities.) */
 int mixed_setup(kifmm_gpu* g) {
   g->mix_lpb = (g->n_s2m > 0 && (double)g->n_src / g->n_s2m < 100.0) ? 4 : 1;
-  if (g->n_s2m > 0)
-    hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
-                       g->src32);
-  HIP_TRY(hipGetLastError());
-  HIP_TRY(hipStreamSynchronize(g->stream));
   return KIFMM_OK;
 }
 
-/* Close pairs of the U-list (k_close_pairs): counts, offsets on the host,
- * then the indices.  An input whose list would exceed 16 pairs per target
- * on average is not listed: the mixed path then keeps the in-loop test
- * (k_p2p_ulist_mixed), the FP64 path the difference form. */
+/* Close pairs of the U-list (k_close_near): a count pass, the exclusive
+ * offsets on the device (k_scan_*), then the fill pass.  Only the total
+ * comes back to the host, to size the index array.  An input whose list
+ * would exceed 16 pairs per target on average is not listed: the mixed path
+ * then keeps the in-loop test (k_p2p_ulist_mixed), the FP64 path the
+ * difference form. */
 int close_list(kifmm_gpu* g) {
-  int* cnt = nullptr;
-  TRY(dev_zero(g, &cnt, (size_t)std::max(g->n_trg, 1)));
-  if (g->n_p2p > 0)
-    hipLaunchKernelGGL(k_close_pairs<false>, dim3(8 * ((g->n_p2p + 7) / 8)), dim3(128), 0, g->stream, g->p2p_items,
-                       g->n_p2p, (g->n_p2p + 7) / 8, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, cnt, nullptr,
-                       nullptr);
-  HIP_TRY(hipGetLastError());
-  std::vector<int> h((size_t)std::max(g->n_trg, 1));
-  HIP_TRY(hipMemcpyAsync(h.data(), cnt, sizeof(int) * h.size(), hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipStreamSynchronize(g->stream));
-  const long long cap = std::max(16LL * g->n_trg, 1LL << 20);
-  std::vector<int> ptr((size_t)g->n_trg + 1, 0);
-  long long tot = 0;
-  for (int i = 0; i < g->n_trg; i++) {
-    if (tot > cap) return KIFMM_OK; /* close_listed stays 0 */
-    ptr[i] = (int)tot;
-    tot += h[i];
+  const long long n = g->n_trg;
+  const int nb = (int)std::max<long long>(1, (n + SCAN_PER - 1) / SCAN_PER);
+  if (nb > SCAN_PER) return KIFMM_OK; /* beyond the scan's reach: not listed */
+  struct Tmp {
+    std::vector<void*> p;
+    ~Tmp() {
+      for (void* q : p) (void)hipFree(q);
+    }
+  } tmp;
+  auto tmp_alloc = [&](size_t bytes) -> void* {
+    void* q = nullptr;
+    if (hipMalloc(&q, std::max<size_t>(bytes, 8)) != hipSuccess) return nullptr;
+    tmp.p.push_back(q);
+    return q;
+  };
+  int* cnt = (int*)tmp_alloc(sizeof(int) * std::max<long long>(n, 1));
+  long long* bsum = (long long*)tmp_alloc(sizeof(long long) * (nb + 1));
+  if (!cnt || !bsum) {
+    set_error("hipMalloc failed for the close-pair search");
+    return KIFMM_ENOMEM;
   }
-  if (tot > cap) return KIFMM_OK;
-  ptr[g->n_trg] = (int)tot;
-  TRY(dev_upload(g, &g->cptr, ptr.data(), ptr.size()));
+  long long* total = bsum + nb;
+  const int per5 = (g->n_p2p5 + 7) / 8;
+  if (g->n_p2p5 > 0)
+    hipLaunchKernelGGL(k_close_near<false>, dim3(8 * per5), dim3(64), 0, g->stream, g->p2p_items5, g->n_p2p5, per5,
+                       g->u_ptr, g->u_idx, g->sbeg, g->snum, g->geo, g->src4, g->trg4, cnt, nullptr, nullptr);
+  HIP_TRY(hipGetLastError());
+  if (n > 0) hipLaunchKernelGGL(k_scan_blocks, dim3(nb), dim3(SCAN_T), 0, g->stream, cnt, n, bsum);
+  hipLaunchKernelGGL(k_scan_tops, dim3(1), dim3(SCAN_T), 0, g->stream, bsum, n > 0 ? nb : 0, total);
+  HIP_TRY(hipGetLastError());
+  long long tot = 0;
+  HIP_TRY(hipMemcpyAsync(&tot, total, sizeof tot, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  const long long cap = std::min<long long>(std::max(16LL * n, 1LL << 20), INT_MAX - 1);
+  if (tot > cap) return KIFMM_OK; /* close_listed stays 0 */
+  TRY(dev_zero(g, &g->cptr, (size_t)n + 1));
   TRY(dev_zero(g, &g->cidx, (size_t)std::max<long long>(tot, 1)));
-  if (g->n_p2p > 0)
-    hipLaunchKernelGGL(k_close_pairs<true>, dim3(8 * ((g->n_p2p + 7) / 8)), dim3(128), 0, g->stream, g->p2p_items,
-                       g->n_p2p, (g->n_p2p + 7) / 8, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, nullptr, g->cptr,
-                       g->cidx);
+  if (n > 0)
+    hipLaunchKernelGGL(k_scan_out, dim3(nb), dim3(SCAN_T), 0, g->stream, cnt, n, bsum, total, g->cptr);
+  if (g->n_p2p5 > 0)
+    hipLaunchKernelGGL(k_close_near<true>, dim3(8 * per5), dim3(64), 0, g->stream, g->p2p_items5, g->n_p2p5, per5,
+                       g->u_ptr, g->u_idx, g->sbeg, g->snum, g->geo, g->src4, g->trg4, nullptr, g->cptr, g->cidx);
   HIP_TRY(hipGetLastError());
   HIP_TRY(hipStreamSynchronize(g->stream));
   g->n_close = tot;
@@ -906,52 +991,67 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   return vlist_phase<NP, double2>(g, g->X, g->Y, g->T, e1, e2, e3);
 }
 
-/* FP64 U-list launch: the Gram form on items of <= P2P_MAXT5 targets, the
- * difference form (no close-pair list) on items of <= P2P_MAXT; the
- * STATIC64 shape on items of <= 64 either way. */
-template <bool MASK, bool GRAM>
-int p2p_fp64(kifmm_gpu* g) {
-  constexpr int G = GRAM ? P2P_G5 : P2P_G;
-  const int4* items = GRAM ? g->p2p_items5 : g->p2p_items;
-  const int n_items = GRAM ? g->n_p2p5 : g->n_p2p, per_xcd = (n_items + 7) / 8;
-  if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
-    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, g->stream, g->p2p_counters, 8, 0);
-    hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, G, GRAM>), dim3(g->persist_grid[GRAM]), dim3(256), 0, g->stream,
-                       items, n_items, per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->p2p_counters,
-                       g->geo, g->trg_gram, g->cptr, g->cidx);
+/* The item list the U-list launch uses: the Gram-form and mixed tau kernels
+ * take items of <= P2P_MAXT5 targets (list 2), the STATIC64 shape items of
+ * <= 64 (list 1), the difference form and the mixed in-loop test items of
+ * <= P2P_MAXT (list 0). */
+int u_list_index(const kifmm_gpu* g) {
+  if (g->precision == KIFMM_PREC_MIXED) return g->close_listed ? 2 : 0;
+  if (g->p2p_form == KIFMM_P2P_STATIC64) return 1;
+  return g->trg_gram ? 2 : 0;
+}
+
+/* U-list over the items of copy-out chunk `chunk` (-1: all items) on stream
+ * st, writing out[t] = u (accum = 0) or out[t] = u + out[t] (accum = 1). */
+template <bool MASK>
+int launch_u(kifmm_gpu* g, hipStream_t st, int chunk, double* out, int accum) {
+  const int l = u_list_index(g);
+  const int4* list = l == 0 ? g->p2p_items : l == 1 ? g->p2p_items64 : g->p2p_items5;
+  const int n_list = l == 0 ? g->n_p2p : l == 1 ? g->n_p2p64 : g->n_p2p5;
+  const int beg = chunk < 0 ? 0 : g->chunk_beg[l][chunk];
+  const int end = chunk < 0 ? n_list : g->chunk_beg[l][chunk + 1];
+  const int n = end - beg, per = (n + 7) / 8;
+  if (n <= 0) return KIFMM_OK;
+  const int4* items = list + beg;
+  if (g->precision == KIFMM_PREC_MIXED) {
+    if (g->close_listed)
+      hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr,
+                         g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx, out, accum);
+    else
+      hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr, g->rng,
+                         g->geo, g->src4, g->trg4, out, accum);
   } else if (g->p2p_form == KIFMM_P2P_STATIC64) {
-    const int per64 = (g->n_p2p64 + 7) / 8;
-    hipLaunchKernelGGL((k_p2p_ulist64<MASK, GRAM>), dim3(8 * per64), dim3(256), 0, g->stream,
-                       g->p2p_items64, g->n_p2p64, per64, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo,
-                       g->trg_gram, g->cptr, g->cidx);
+    if (g->trg_gram)
+      hipLaunchKernelGGL((k_p2p_ulist64<MASK, true>), dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr,
+                         g->rng, g->src4, g->trg4, out, g->geo, g->trg_gram, g->cptr, g->cidx, accum);
+    else
+      hipLaunchKernelGGL((k_p2p_ulist64<MASK, false>), dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr,
+                         g->rng, g->src4, g->trg4, out, g->geo, g->trg_gram, g->cptr, g->cidx, accum);
+  } else if (g->p2p_form == KIFMM_P2P_PERSISTENT) {
+    int* counters = g->p2p_counters + 8 * std::max(chunk, 0);
+    hipLaunchKernelGGL(k_fill_int, dim3(1), dim3(64), 0, st, counters, 8, 0);
+    if (g->trg_gram)
+      hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, P2P_G5, true>), dim3(g->persist_grid[1]), dim3(256), 0, st, items,
+                         n, per, g->rng_ptr, g->rng, g->src4, g->trg4, out, counters, g->geo, g->trg_gram, g->cptr,
+                         g->cidx, accum);
+    else
+      hipLaunchKernelGGL((k_p2p_ulist_persist<MASK, P2P_G, false>), dim3(g->persist_grid[0]), dim3(256), 0, st, items,
+                         n, per, g->rng_ptr, g->rng, g->src4, g->trg4, out, counters, g->geo, g->trg_gram, g->cptr,
+                         g->cidx, accum);
   } else {
-    hipLaunchKernelGGL((k_p2p_ulist<MASK, G, GRAM>), dim3(8 * per_xcd), dim3(256), 0, g->stream, items, n_items,
-                       per_xcd, g->rng_ptr, g->rng, g->src4, g->trg4, g->pot, g->geo, g->trg_gram, g->cptr,
-                       g->cidx);
+    if (g->trg_gram)
+      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G5, true>), dim3(8 * per), dim3(256), 0, st, items, n, per,
+                         g->rng_ptr, g->rng, g->src4, g->trg4, out, g->geo, g->trg_gram, g->cptr, g->cidx, accum);
+    else
+      hipLaunchKernelGGL((k_p2p_ulist<MASK, P2P_G, false>), dim3(8 * per), dim3(256), 0, st, items, n, per,
+                         g->rng_ptr, g->rng, g->src4, g->trg4, out, g->geo, g->trg_gram, g->cptr, g->cidx, accum);
   }
   LAUNCH_CHECK();
   return KIFMM_OK;
 }
 
-template <bool MASK>
-int point_phases_u(kifmm_gpu* g) {
-  if (g->n_p2p > 0) {
-    const int per_xcd = (g->n_p2p + 7) / 8;
-    if (g->precision == KIFMM_PREC_MIXED && g->close_listed) {
-      const int per5 = (g->n_p2p5 + 7) / 8; /* items of <= P2P_MAXT5 targets */
-      hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per5), dim3(256), 0, g->stream, g->p2p_items5,
-                         g->n_p2p5, per5, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx, g->pot);
-    }
-    else if (g->precision == KIFMM_PREC_MIXED)
-      hipLaunchKernelGGL(k_p2p_ulist_mixed<MASK>, dim3(8 * per_xcd), dim3(256), 0, g->stream, g->p2p_items,
-                         g->n_p2p, per_xcd, g->rng_ptr, g->rng, g->geo, g->src4, g->trg4, g->pot);
-    else if (g->trg_gram)
-      return p2p_fp64<MASK, true>(g);
-    else
-      return p2p_fp64<MASK, false>(g);
-  }
-  LAUNCH_CHECK();
-  return KIFMM_OK;
+int launch_u(kifmm_gpu* g, hipStream_t st, int chunk, double* out, int accum) {
+  return g->coincident ? launch_u<true>(g, st, chunk, out, accum) : launch_u<false>(g, st, chunk, out, accum);
 }
 
 template <bool MASK>
@@ -978,51 +1078,60 @@ int point_phases_wx(kifmm_gpu* g, unsigned mask, hipEvent_t ew) {
   return KIFMM_OK;
 }
 
-/* The launch sequence of one evaluation on the handle's stream.  Phase
- * events are recorded only outside graph capture. */
-int enqueue(kifmm_gpu* g, unsigned mask) {
+/* Phase events (recorded outside graph capture): E_START before the
+ * sequence, then one after each phase in launch order. */
+enum { E_START = 0, E_UP, E_FFT, E_TRANS, E_HAD, E_IFFT, E_W, E_X, E_DOWN, E_U, E_END };
+
+/* device -> host copy of potentials [b, e) to dst + b */
+int copy_out(kifmm_gpu* g, hipStream_t st, double* dst, int b, int e) {
+  if (e > b)
+    HIP_TRY(hipMemcpyAsync(dst + b, g->pot + b, sizeof(double) * (size_t)(e - b), hipMemcpyDeviceToHost, st));
+  return KIFMM_OK;
+}
+
+/* The launch sequence of one evaluation.  Order: up, V, W, X, down, then
+ * the U-list, so that the U-list's accumulating write leaves each target
+ * final and its chunk can go to the host (dst, when non-null) on the copy
+ * stream while the later chunks compute; the chunks alternate between the
+ * handle's two compute streams, so one chunk's tail overlaps the next
+ * one's start.  With the overlap layout the U-list instead runs from the
+ * start on the second stream into potU, alongside everything else, and
+ * k_join_pot adds it in at the end (the same sum, operand for operand). */
+int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
   hipEvent_t* ev = g->ev;
   const int lde = g->ld_e, lduc = g->ld_uc;
-  /* zero what this call would otherwise leave stale (a fill kernel, so the
-   * sequence stays capturable).  The U-list assigns pot for every target
-   * (each target leaf has items), the up pass assigns uden for every source
-   * node of depth >= 2 (the only ones read; the others are zero from create
-   * time and never written), and the down pass assigns dden for every
-   * target node of depth >= 2 (the only ones read), so each is cleared only
-   * when its phase is not in the mask; dchk is accumulated into (V, X, L2L)
-   * and always cleared. */
-  if (!(mask & KIFMM_PHASE_U)) TRY(fill(g, g->pot, (size_t)g->n_trg));
+  /* zero what this call accumulates into (fill kernels, so the sequence
+   * stays capturable): pot (W, X, L2T and the U-list add into it) and dchk
+   * (V, X, L2L) always; uden and dden only when their phase, which assigns
+   * every entry that is read, is left out */
+  TRY(fill(g, g->pot, (size_t)g->n_trg));
   if (!(mask & KIFMM_PHASE_UP)) TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
   TRY(fill(g, g->dchk, (size_t)g->n_nodes * lde));
   if (!(mask & KIFMM_PHASE_DOWN)) TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
-  /* KIFMM_GPU_OVERLAP: the U-list forks onto the side stream here and runs
-   * alongside up and V.  It reads only points, lists and create-time data
-   * and writes pot with '=' (the pot fill above is ordered before the
-   * fork); nothing on this stream touches pot until the join, placed before
-   * W, X and down, which add into pot. */
-  const bool split = g->overlap && (mask & KIFMM_PHASE_U);
-  if (split) {
+  const bool use_u = (mask & KIFMM_PHASE_U) != 0;
+  const bool side_u = g->overlap && use_u;
+  if (side_u) {
     HIP_TRY(hipEventRecord(g->fork_ev, g->stream));
     HIP_TRY(hipStreamWaitEvent(g->side, g->fork_ev, 0));
     if (!g->capturing) HIP_TRY(hipEventRecord(g->su0, g->side));
-    hipStream_t main_stream = g->stream;
-    g->stream = g->side;
-    const int r = g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g);
-    g->stream = main_stream;
-    TRY(r);
+    TRY(launch_u(g, g->side, -1, g->potU, 0));
     if (!g->capturing) HIP_TRY(hipEventRecord(g->su1, g->side));
     HIP_TRY(hipEventRecord(g->join_ev, g->side));
   }
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
-    if (g->precision == KIFMM_PREC_MIXED) /* src32 was written at create time (mixed_setup) */
+    if (g->precision == KIFMM_PREC_MIXED) {
+      /* FP32 leaf-local copies of the current sources and densities */
+      hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
+                         g->src32);
       hipLaunchKernelGGL(g->mix_lpb == 4 ? k_s2m_mixed<4> : k_s2m_mixed<1>, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb),
                          dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->n_s2m, g->geo, g->surf_uc32,
                          g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
-    else
+    } else {
       hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
                          g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+    }
     LAUNCH_CHECK();
     TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
@@ -1030,38 +1139,33 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
     TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
     TRY(gemm(g, g->lv[d].m2m_b, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
-  REC(ev[1]);
-  /* U-list */
-  if ((mask & KIFMM_PHASE_U) && !split) TRY(g->coincident ? point_phases_u<true>(g) : point_phases_u<false>(g));
-  REC(ev[2]);
+  REC(ev[E_UP]);
   /* V-list */
   if (mask & KIFMM_PHASE_V) switch (g->np) {
-    case 3: TRY(vlist_phase<3>(g, ev[3], ev[4], ev[5])); break;
-    case 4: TRY(vlist_phase<4>(g, ev[3], ev[4], ev[5])); break;
-    case 6: TRY(vlist_phase<6>(g, ev[3], ev[4], ev[5])); break;
+    case 3: TRY(vlist_phase<3>(g, ev[E_FFT], ev[E_TRANS], ev[E_HAD])); break;
+    case 4: TRY(vlist_phase<4>(g, ev[E_FFT], ev[E_TRANS], ev[E_HAD])); break;
+    case 6: TRY(vlist_phase<6>(g, ev[E_FFT], ev[E_TRANS], ev[E_HAD])); break;
     default: set_error("NP must be 3, 4 or 6 on the GPU"); return KIFMM_EUNSUP;
   }
   else {
-    REC(ev[3]);
-    REC(ev[4]);
-    REC(ev[5]);
+    REC(ev[E_FFT]);
+    REC(ev[E_TRANS]);
+    REC(ev[E_HAD]);
   }
-  if (split) HIP_TRY(hipStreamWaitEvent(g->stream, g->join_ev, 0));
-  REC(ev[6]);
+  REC(ev[E_IFFT]);
   /* W and X */
   if (mask & (KIFMM_PHASE_W | KIFMM_PHASE_X))
-    TRY(g->coincident ? point_phases_wx<true>(g, mask, ev[7]) : point_phases_wx<false>(g, mask, ev[7]));
+    TRY(g->coincident ? point_phases_wx<true>(g, mask, ev[E_W]) : point_phases_wx<false>(g, mask, ev[E_W]));
   else
-    REC(ev[7]);
-  REC(ev[8]);
+    REC(ev[E_W]);
+  REC(ev[E_X]);
   /* down: per level L2L (depth >= 3) then DC2DE (depth >= 2), then L2T */
   for (int d = 2; d <= g->max_depth && (mask & KIFMM_PHASE_DOWN); d++) {
     if (d >= 3)
       TRY(gemm(g, g->lv[d - 1].l2l, 8 * lde, lde, g->DE2DC, g->lda_l2l, g->dden, g->dchk, 1));
     TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
-  if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0)
-  {
+  if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0) {
     if (g->precision == KIFMM_PREC_MIXED)
       hipLaunchKernelGGL(g->mix_lpb == 4 ? k_l2t_mixed<4> : k_l2t_mixed<1>, dim3((g->n_l2t + g->mix_lpb - 1) / g->mix_lpb),
                          dim3(64), 0, g->stream, g->l2t_items, g->n_l2t, g->geo, g->surf_de32,
@@ -1071,18 +1175,76 @@ int enqueue(kifmm_gpu* g, unsigned mask) {
                          g->n_e, g->dden, lde, g->trg4, g->pot);
   }
   LAUNCH_CHECK();
-  REC(ev[9]);
+  REC(ev[E_DOWN]);
+  const int n = g->n_trg;
+  if (side_u) {
+    HIP_TRY(hipStreamWaitEvent(g->stream, g->join_ev, 0));
+    const int blocks = (int)std::min<size_t>(4096, ((size_t)n + 255) / 256);
+    if (n > 0) hipLaunchKernelGGL(k_join_pot, dim3(std::max(blocks, 1)), dim3(256), 0, g->stream, g->pot, g->potU, (size_t)n);
+    LAUNCH_CHECK();
+    REC(ev[E_U]);
+    if (dst) TRY(copy_out(g, g->stream, dst, 0, n));
+  } else if (use_u) {
+    const int nch = dst ? (int)g->cuts.size() - 1 : 1;
+    if (nch > 1) {
+      HIP_TRY(hipEventRecord(g->far_ev, g->stream));
+      HIP_TRY(hipStreamWaitEvent(g->side, g->far_ev, 0));
+    }
+    for (int c = 0; c < nch; c++) {
+      hipStream_t st = (c & 1) ? g->side : g->stream;
+      TRY(launch_u(g, st, nch > 1 ? c : -1, g->pot, 1));
+      if (dst) {
+        HIP_TRY(hipEventRecord(g->ch_ev[c], st));
+        HIP_TRY(hipStreamWaitEvent(g->cstream, g->ch_ev[c], 0));
+        TRY(copy_out(g, g->cstream, dst, nch > 1 ? g->cuts[c] : 0, nch > 1 ? g->cuts[c + 1] : n));
+        if (!g->capturing) HIP_TRY(hipEventRecord(g->cd_ev[c], g->cstream));
+      }
+    }
+    if (nch > 1) {
+      HIP_TRY(hipEventRecord(g->sdone_ev, g->side));
+      HIP_TRY(hipStreamWaitEvent(g->stream, g->sdone_ev, 0));
+    }
+    REC(ev[E_U]);
+    if (dst) {
+      HIP_TRY(hipEventRecord(g->cdone_ev, g->cstream));
+      HIP_TRY(hipStreamWaitEvent(g->stream, g->cdone_ev, 0));
+    }
+  } else {
+    REC(ev[E_U]);
+    if (dst) TRY(copy_out(g, g->stream, dst, 0, n));
+  }
   return KIFMM_OK;
+}
+
+/* pinned potentials -> the caller's buffer, a few host threads */
+void host_copy(double* to, const double* from, size_t b, size_t e) {
+  const size_t n = e - b;
+  const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / (1 << 16)));
+  if (nt == 1) {
+    std::memcpy(to + b, from + b, sizeof(double) * n);
+    return;
+  }
+  std::vector<std::thread> th;
+  for (int k = 0; k < nt; k++) {
+    const size_t x = b + n * k / nt, y = b + n * (k + 1) / nt;
+    th.emplace_back([=] { std::memcpy(to + x, from + x, sizeof(double) * (y - x)); });
+  }
+  for (auto& t : th) t.join();
 }
 
 int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* times) {
   hipEvent_t* ev = g->ev;
   HIP_TRY(hipSetDevice(g->dev));
   const bool graph = g->use_graph && mask == KIFMM_PHASE_ALL;
-  if (graph && !g->gexec) {
+  /* the copy lands in the handle's pinned buffer (the caller's own buffer
+   * when it is that one; otherwise one host pass follows); without pinned
+   * memory, eagerly into the caller's buffer, or after a graph */
+  double* dst = pot_leaf ? (g->pinned ? g->pinned : (graph ? nullptr : pot_leaf)) : nullptr;
+  const int gi = dst ? 1 : 0;
+  if (graph && !g->gexec[gi]) {
     g->capturing = true;
     HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    const int r = enqueue(g, mask);
+    const int r = enqueue(g, mask, dst);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(g->stream, &gr);
     g->capturing = false;
@@ -1091,63 +1253,64 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
       return r;
     }
     HIP_TRY(e);
-    g->graph = gr;
-    HIP_TRY(hipGraphInstantiate(&g->gexec, gr, nullptr, nullptr, 0));
+    g->graph[gi] = gr;
+    HIP_TRY(hipGraphInstantiate(&g->gexec[gi], gr, nullptr, nullptr, 0));
   }
-  HIP_TRY(hipEventRecord(ev[0], g->stream));
+  HIP_TRY(hipEventRecord(ev[E_START], g->stream));
   if (graph) {
-    HIP_TRY(hipGraphLaunch(g->gexec, g->stream));
-    for (int i = 1; i <= 9; i++) HIP_TRY(hipEventRecord(ev[i], g->stream));
+    HIP_TRY(hipGraphLaunch(g->gexec[gi], g->stream));
+    for (int i = E_UP; i <= E_U; i++) HIP_TRY(hipEventRecord(ev[i], g->stream));
   } else {
-    TRY(enqueue(g, mask));
+    TRY(enqueue(g, mask, dst));
   }
-  if (pot_leaf)
-    HIP_TRY(hipMemcpyAsync(g->pinned ? g->pinned : pot_leaf, g->pot, sizeof(double) * g->n_trg,
-                           hipMemcpyDeviceToHost, g->stream));
-  HIP_TRY(hipEventRecord(ev[10], g->stream));
-  HIP_TRY(hipEventSynchronize(ev[10]));
+  if (pot_leaf && !dst) /* a graph handle without pinned memory */
+    HIP_TRY(hipMemcpyAsync(pot_leaf, g->pot, sizeof(double) * g->n_trg, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipEventRecord(ev[E_END], g->stream));
   double host_copy_ms = 0.0;
-  if (pot_leaf && g->pinned) { /* pinned staging -> caller's buffer, a few host threads */
-    const double t0 = kifmm::wall_seconds();
-    const size_t n = (size_t)g->n_trg;
-    const int nt = (int)std::min<size_t>(8, std::max<size_t>(1, n / (1 << 16)));
-    std::vector<std::thread> th;
-    for (int k = 0; k < nt; k++) {
-      const size_t b = n * k / nt, e = n * (k + 1) / nt;
-      th.emplace_back([=] { std::memcpy(pot_leaf + b, g->pinned + b, sizeof(double) * (e - b)); });
+  if (pot_leaf && dst == g->pinned && pot_leaf != g->pinned) {
+    const int nch = (int)g->cuts.size() - 1;
+    const bool pipelined = !graph && !g->overlap && (mask & KIFMM_PHASE_U) && nch > 1;
+    double t0 = 0.0;
+    if (pipelined) { /* each chunk as soon as its copy is done */
+      for (int c = 0; c < nch; c++) {
+        HIP_TRY(hipEventSynchronize(g->cd_ev[c]));
+        if (c == nch - 1) t0 = kifmm::wall_seconds();
+        host_copy(pot_leaf, g->pinned, (size_t)g->cuts[c], (size_t)g->cuts[c + 1]);
+      }
+    } else {
+      HIP_TRY(hipEventSynchronize(ev[E_END]));
+      t0 = kifmm::wall_seconds();
+      host_copy(pot_leaf, g->pinned, 0, (size_t)g->n_trg);
     }
-    for (auto& x : th) x.join();
-    host_copy_ms = (kifmm::wall_seconds() - t0) * 1e3;
+    host_copy_ms = (kifmm::wall_seconds() - t0) * 1e3; /* what is left after the device work */
   }
+  HIP_TRY(hipEventSynchronize(ev[E_END]));
   if (times) {
-    float ms[11] = {0};
-    for (int i = 1; i <= 10; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
+    float ms[E_END + 1] = {0};
+    for (int i = E_UP; i <= E_END; i++) HIP_TRY(hipEventElapsedTime(&ms[i], ev[i - 1], ev[i]));
     float tot = 0;
-    HIP_TRY(hipEventElapsedTime(&tot, ev[0], ev[10]));
+    HIP_TRY(hipEventElapsedTime(&tot, ev[E_START], ev[E_END]));
     std::memset(times, 0, sizeof *times);
-    times->up = ms[1];
-    times->ulist = ms[2];
-    times->p2p_kernel = ms[2];
-    times->v_fft_src = ms[3];
-    times->v_trans = ms[4];
-    times->v_hadamard = ms[5];
-    times->v_ifft = ms[6];
-    times->vlist = ms[3] + ms[4] + ms[5] + ms[6];
-    times->wlist = ms[7];
-    times->xlist = ms[8];
-    times->down = ms[9];
-    if (!graph && g->overlap && (mask & KIFMM_PHASE_U)) { /* the U-list ran on the side stream */
-      float mu = 0;
-      HIP_TRY(hipEventElapsedTime(&mu, g->su0, g->su1));
-      times->ulist = times->p2p_kernel = mu;
+    if (!graph) {
+      times->up = ms[E_UP];
+      times->v_fft_src = ms[E_FFT];
+      times->v_trans = ms[E_TRANS];
+      times->v_hadamard = ms[E_HAD];
+      times->v_ifft = ms[E_IFFT];
+      times->vlist = ms[E_FFT] + ms[E_TRANS] + ms[E_HAD] + ms[E_IFFT];
+      times->wlist = ms[E_W];
+      times->xlist = ms[E_X];
+      times->down = ms[E_DOWN];
+      times->ulist = times->p2p_kernel = ms[E_U];
+      if (g->overlap && (mask & KIFMM_PHASE_U)) { /* the U-list ran on the side stream */
+        float mu = 0;
+        HIP_TRY(hipEventElapsedTime(&mu, g->su0, g->su1));
+        times->ulist = times->p2p_kernel = mu;
+      }
     }
-    times->copy = ms[10] + host_copy_ms;
+    /* the copy left after the last kernel (the rest overlaps the U-list) */
+    times->copy = (graph ? 0.0 : ms[E_END]) + host_copy_ms;
     times->total = tot + host_copy_ms;
-    if (graph) { /* one graph launch: no per-phase events inside it */
-      std::memset(times, 0, sizeof *times);
-      times->copy = ms[10] + host_copy_ms;
-      times->total = tot + host_copy_ms;
-    }
   }
   return KIFMM_OK;
 }
@@ -1182,15 +1345,22 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
     return KIFMM_EINVAL;
   }
   /* every index the kernels derive is checked on the host before any device call */
-  {
-    const int pr = kifmm_gpu_plan(tree, ops, nullptr);
+  const double t_plan0 = kifmm::wall_seconds();
+  Plan P;
+  try {
+    const int pr = make_plan(tree, ops, P);
     if (pr != KIFMM_OK) return pr;
+  } catch (const std::bad_alloc&) {
+    set_error("host allocation failed in kifmm_gpu_create");
+    return KIFMM_ENOMEM;
   }
+  const double t_plan1 = kifmm::wall_seconds();
   kifmm_gpu* g = new (std::nothrow) kifmm_gpu();
   if (!g) {
     set_error("host allocation failed");
     return KIFMM_ENOMEM;
   }
+  g->create_ms[0] = (t_plan1 - t_plan0) * 1e3;
   g->dev = opts ? opts->device : 0;
   g->precision = opts ? opts->precision : KIFMM_PREC_FP64;
   g->use_graph = opts ? (opts->flags & KIFMM_GPU_GRAPH) != 0 : 0;
@@ -1220,19 +1390,40 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
   g->overlap = opts ? (opts->flags & KIFMM_GPU_OVERLAP) != 0 : 0;
   /* the second stream exists on every handle: the layout is a tuning choice */
   if (hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) != hipSuccess ||
+      hipStreamCreateWithFlags(&g->cstream, hipStreamNonBlocking) != hipSuccess ||
       hipEventCreateWithFlags(&g->fork_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreateWithFlags(&g->join_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g->far_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g->sdone_ev, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&g->cdone_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&g->su0) != hipSuccess || hipEventCreate(&g->su1) != hipSuccess) {
-    set_error("kifmm_gpu_create: second stream or its events could not be created");
+    set_error("kifmm_gpu_create: second stream, copy stream or their events could not be created");
     kifmm_gpu_destroy(g);
     return KIFMM_EDEVICE;
   }
+  g->ch_ev.assign(P.cuts.size(), nullptr);
+  g->cd_ev.assign(P.cuts.size(), nullptr);
+  for (size_t c = 0; c < P.cuts.size(); c++)
+    if (hipEventCreateWithFlags(&g->ch_ev[c], hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&g->cd_ev[c], hipEventDisableTiming) != hipSuccess) {
+      set_error("kifmm_gpu_create: copy-out events could not be created");
+      kifmm_gpu_destroy(g);
+      return KIFMM_EDEVICE;
+    }
   try {
-    r = build(g, tree, ops);
+    const double t0 = kifmm::wall_seconds();
+    r = build(g, tree, ops, P);
+    if (r == KIFMM_OK) r = hipStreamSynchronize(g->stream) == hipSuccess ? KIFMM_OK : KIFMM_EDEVICE;
+    const double t1 = kifmm::wall_seconds();
+    if (r == KIFMM_OK && !(opts && (opts->flags & KIFMM_GPU_NO_CLOSE_LIST))) r = close_list(g);
+    const double t2 = kifmm::wall_seconds();
     if (r == KIFMM_OK) r = trans_setup(g);
     if (r == KIFMM_OK && g->precision == KIFMM_PREC_MIXED) r = mixed_setup(g);
-    if (r == KIFMM_OK && !(opts && (opts->flags & KIFMM_GPU_NO_CLOSE_LIST))) r = close_list(g);
     if (r == KIFMM_OK && g->precision == KIFMM_PREC_FP64 && g->close_listed) r = gram_setup(g);
+    const double t3 = kifmm::wall_seconds();
+    g->create_ms[1] = (t1 - t0) * 1e3;
+    g->create_ms[2] = (t2 - t1) * 1e3;
+    g->create_ms[3] = (t3 - t2) * 1e3;
   } catch (const std::bad_alloc&) {
     set_error("host allocation failed in kifmm_gpu_create");
     r = KIFMM_ENOMEM;
@@ -1250,6 +1441,7 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
     kifmm_gpu_destroy(g);
     return KIFMM_EDEVICE;
   }
+  g->create_ms[4] = (kifmm::wall_seconds() - t_plan0) * 1e3;
   *out = g;
   return KIFMM_OK;
 }
@@ -1275,13 +1467,22 @@ int kifmm_gpu_destroy(kifmm_gpu* g) {
   (void)hipSetDevice(g->dev);
   if (g->stream) (void)hipStreamSynchronize(g->stream);
   if (g->pinned) (void)hipHostFree(g->pinned);
-  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
+  if (g->side) (void)hipStreamSynchronize(g->side);
+  if (g->cstream) (void)hipStreamSynchronize(g->cstream);
+  for (int k = 0; k < 2; k++) {
+    if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
+    if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
+  }
   for (void* p : g->allocs) (void)hipFree(p);
   for (auto& e : g->ev)
     if (e) (void)hipEventDestroy(e);
-  for (hipEvent_t e : {g->fork_ev, g->join_ev, g->su0, g->su1})
+  for (hipEvent_t e : {g->fork_ev, g->join_ev, g->su0, g->su1, g->far_ev, g->sdone_ev, g->cdone_ev})
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : g->ch_ev)
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : g->cd_ev)
+    if (e) (void)hipEventDestroy(e);
+  if (g->cstream) (void)hipStreamDestroy(g->cstream);
   if (g->side) (void)hipStreamDestroy(g->side);
   if (g->stream) (void)hipStreamDestroy(g->stream);
   delete g;
@@ -1385,10 +1586,12 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
   g->had_fb = t->hadamard_fb;
   g->had_form = t->hadamard_form;
   g->overlap = t->overlap;
-  if (g->gexec) (void)hipGraphExecDestroy(g->gexec);
-  if (g->graph) (void)hipGraphDestroy(g->graph);
-  g->gexec = nullptr;
-  g->graph = nullptr;
+  for (int k = 0; k < 2; k++) {
+    if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
+    if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
+    g->gexec[k] = nullptr;
+    g->graph[k] = nullptr;
+  }
   return KIFMM_OK;
 }
 
@@ -1456,7 +1659,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     rc = kifmm_gpu_set_tuning(g, &c);
     kifmm_phase_times pt{};
     for (int r = 0; r <= reps && rc == KIFMM_OK; r++) {
-      rc = evaluate(g, KIFMM_PHASE_ALL, nullptr, &pt);
+      rc = evaluate(g, KIFMM_PHASE_ALL, g->pinned, &pt); /* with the copy-out, as the layouts differ there */
       if (r == 1 || (r > 1 && pt.total < t[9 + l])) t[9 + l] = pt.total; /* run 0 warms up */
     }
     if (rc == KIFMM_OK && t[9 + l] < t[9 + best_l]) best_l = l;
@@ -1475,7 +1678,57 @@ int kifmm_gpu_work_counts(const kifmm_gpu* g, kifmm_work_counts* out) {
   if (!g || !out) return KIFMM_EINVAL;
   *out = g->counts;
   out->close_pairs = g->close_listed ? (double)g->n_close : -1.0;
+  out->copy_chunks = (double)std::max<size_t>(g->cuts.size(), 2) - 1.0;
   return KIFMM_OK;
+}
+
+double* kifmm_gpu_output(kifmm_gpu* g) { return g ? g->pinned : nullptr; }
+
+int kifmm_gpu_set_densities(kifmm_gpu* g, const double* den_leaf) {
+  if (!g || (!den_leaf && g->n_src > 0)) {
+    set_error("kifmm_gpu_set_densities: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  if (g->n_src == 0) return KIFMM_OK;
+  HIP_TRY(hipSetDevice(g->dev));
+  HIP_TRY(hipMemcpyAsync(g->den_stage, den_leaf, sizeof(double) * (size_t)g->n_src, hipMemcpyHostToDevice,
+                         g->stream));
+  const int blocks = (int)std::min<size_t>(4096, ((size_t)g->n_src + 255) / 256);
+  hipLaunchKernelGGL(k_set_w, dim3(blocks), dim3(256), 0, g->stream, g->src4, g->den_stage, (size_t)g->n_src);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipStreamSynchronize(g->stream)); /* den_leaf may be released on return */
+  return KIFMM_OK;
+}
+
+int kifmm_gpu_create_times(const kifmm_gpu* g, double* ms, int n) {
+  if (!g || !ms || n < 0) {
+    set_error("kifmm_gpu_create_times: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  for (int k = 0; k < std::min(n, 5); k++) ms[k] = g->create_ms[k];
+  return KIFMM_OK;
+}
+
+long kifmm_gpu_close_list(const kifmm_gpu* g, int* ptr, long n_ptr, int* idx, long n_idx) {
+  if (!g || n_ptr < 0 || n_idx < 0 || (n_ptr && !ptr) || (n_idx && !idx)) {
+    set_error("kifmm_gpu_close_list: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  if (!g->close_listed) { /* KIFMM_GPU_NO_CLOSE_LIST, or too many pairs */
+    set_error("kifmm_gpu_close_list: this handle has no close-pair list");
+    return KIFMM_EUNSUP;
+  }
+  if (n_ptr > (long)g->n_trg + 1 || n_idx > g->n_close) {
+    set_error("kifmm_gpu_close_list: count exceeds the list");
+    return KIFMM_EINVAL;
+  }
+  if (hipSetDevice(g->dev) != hipSuccess || hipStreamSynchronize(g->stream) != hipSuccess ||
+      (n_ptr && hipMemcpy(ptr, g->cptr, sizeof(int) * n_ptr, hipMemcpyDeviceToHost) != hipSuccess) ||
+      (n_idx && hipMemcpy(idx, g->cidx, sizeof(int) * n_idx, hipMemcpyDeviceToHost) != hipSuccess)) {
+    set_error("kifmm_gpu_close_list: copy failed");
+    return KIFMM_EDEVICE;
+  }
+  return (long)g->n_close;
 }
 
 int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long count) {

@@ -12,7 +12,7 @@
  *   target binning fmm/input.cc:386-460
  *   lists          fmm/input.cc:303-381
  *   gather         fmm/evaluate.cc:13-96   (leaf ranges assigned in BFS order)
- *   operators      fmm/trans.cc:73-326, pinv fmm/vecmatop.cc:34-78 + fmm/svdrep.cc:10-66
+ *   operators      fmm/trans.cc:13-264, pinv fmm/vecmatop.cc:34-78 + fmm/svdrep.cc:10-66
  *   validate       fmm/driver.cc:141-206
  * The implementation is this library's own: index-range octree with a
  * stable counting sort per node (no per-node vectors), sorted-vector lists,
@@ -437,7 +437,7 @@ struct Surface {
   std::vector<double> x, y, z;
 };
 
-/* fmm/trans.cc:91-120: boundary nodes of an np^3 grid on [-R,R]^3, x slowest */
+/* fmm/trans.cc:31-62: boundary nodes of an np^3 grid on [-R,R]^3, x slowest */
 Surface surface(int np, double R) {
   Surface s;
   const double step = 2.0 / (np - 1);
@@ -570,12 +570,12 @@ void build_operators(kifmm_problem* p) {
   p->t_ops = wall_seconds() - t0;
 }
 
-/* UC2UE, UE2UC, DC2DE, DE2DC for p->np (fmm/trans.cc:203-326) */
+/* UC2UE, UE2UC, DC2DE, DE2DC for p->np (fmm/trans.cc:143-264) */
 void compute_operators(kifmm_problem* p) {
   const int np = p->np;
   const Surface ue = surface(np, 1.0), uc = surface(np + 2, 3.0), de = surface(np, 3.0), dc = surface(np, 1.0);
   const int ne = ue.n, nuc = uc.n;
-  const double eps = std::pow(0.1, np + 1); /* alt(), fmm/trans.cc:180-184 */
+  const double eps = std::pow(0.1, np + 1); /* alt(), fmm/trans.cc:120-124 */
   p->uc2ue.assign((size_t)ne * nuc, 0.0);
   p->ue2uc.assign((size_t)8 * nuc * ne, 0.0);
   p->dc2de.assign((size_t)ne * ne, 0.0);

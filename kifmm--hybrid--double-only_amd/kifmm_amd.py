@@ -92,7 +92,7 @@ class PhaseTimes(C.Structure):
 class WorkCounts(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "u_pairs", "s2m_pairs", "l2t_pairs", "w_pairs", "x_pairs", "v_entries", "v_cmac", "v_cmac_ref",
-        "gemm_flops", "fft_bytes", "hadamard_bytes", "close_pairs")] + [("reserved", C.c_double * 4)]
+        "gemm_flops", "fft_bytes", "hadamard_bytes", "close_pairs", "copy_chunks")] + [("reserved", C.c_double * 3)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}
@@ -139,6 +139,10 @@ _sig("kifmm_gpu_plan", C.c_int, [C.POINTER(TreeView), C.POINTER(Operators), C.PO
 _sig("kifmm_gpu_set_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
 _sig("kifmm_gpu_get_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
 _sig("kifmm_gpu_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(GpuTuning), _dp, C.c_int])
+_sig("kifmm_gpu_output", _dp, [C.c_void_p])
+_sig("kifmm_gpu_set_densities", C.c_int, [C.c_void_p, _dp])
+_sig("kifmm_gpu_create_times", C.c_int, [C.c_void_p, _dp, C.c_int])
+_sig("kifmm_gpu_close_list", C.c_long, [C.c_void_p, _ip, C.c_long, _ip, C.c_long])
 
 
 def last_error() -> str:
@@ -353,6 +357,7 @@ class GpuEvaluator:
 
     def close(self):
         if getattr(self, "_h", None):
+            self._out = None  # the pinned buffer goes with the handle
             lib.kifmm_gpu_destroy(self._h)
             self._h = None
 
@@ -362,18 +367,72 @@ class GpuEvaluator:
         except Exception:
             pass
 
+    def output(self):
+        """The handle's pinned host buffer of potentials (gathered order) as a
+        numpy view, overwritten by every evaluate(copy="view"); None without
+        pinned memory."""
+        if getattr(self, "_out", None) is None:
+            p = lib.kifmm_gpu_output(self._h)
+            self._out = np.ctypeslib.as_array(p, shape=(self.n_trg,)) if p and self.n_trg else None
+        return self._out
+
     def evaluate(self, copy=True):
-        """Returns (potentials in gathered order or None, phase times in ms)."""
+        """Returns (potentials in gathered order or None, phase times in ms).
+        copy=True: the potentials come back into the handle's pinned buffer
+        (chunk by chunk while the U-list still runs) and a numpy copy of them
+        is returned; copy="view": the pinned buffer itself is returned (no
+        host pass; the next evaluation overwrites it); copy=False: they stay
+        on the device."""
         t = PhaseTimes()
-        pot = np.zeros(self.n_trg) if copy else None
-        _check(lib.kifmm_gpu_evaluate(self._h, _dptr(pot) if copy else None, C.byref(t)))
-        return pot, t.as_dict()
+        out = self.output() if copy else None
+        if copy and out is None:  # no pinned memory: straight into a numpy array
+            pot = np.zeros(self.n_trg)
+            _check(lib.kifmm_gpu_evaluate(self._h, _dptr(pot), C.byref(t)))
+            return pot, t.as_dict()
+        _check(lib.kifmm_gpu_evaluate(self._h, _dptr(out) if copy else None, C.byref(t)))
+        if not copy:
+            return None, t.as_dict()
+        return (out if copy == "view" else out.copy()), t.as_dict()
+
+    def evaluate_into(self, pot):
+        """kifmm_gpu_evaluate into a caller array (float64, n_trg): the pinned
+        copy, then one host pass into it."""
+        assert pot.dtype == np.float64 and pot.flags.c_contiguous and pot.size == self.n_trg
+        t = PhaseTimes()
+        _check(lib.kifmm_gpu_evaluate(self._h, _dptr(pot), C.byref(t)))
+        return t.as_dict()
+
+    def set_densities(self, den_leaf):
+        """New source densities in gathered source order (kifmm_gpu_set_densities)."""
+        den = np.ascontiguousarray(den_leaf, dtype=np.float64)
+        assert den.size == self._view.n_src
+        _check(lib.kifmm_gpu_set_densities(self._h, _dptr(den)))
+
+    def create_times(self):
+        """Wall ms of kifmm_gpu_create: host plan, upload, close-pair search,
+        other device set-up, whole call."""
+        t = np.zeros(5)
+        _check(lib.kifmm_gpu_create_times(self._h, _dptr(t), 5))
+        return dict(zip(("plan", "upload", "close_pairs", "device_setup", "total"), t.tolist()))
+
+    def close_list(self):
+        """(offsets [n_trg + 1], source indices) of the close-pair list, or None."""
+        n = lib.kifmm_gpu_close_list(self._h, None, 0, None, 0)
+        if n < 0:
+            if n == -4:
+                return None
+            _check(n)
+        ptr = np.zeros(self.n_trg + 1, dtype=np.int32)
+        idx = np.zeros(max(n, 1), dtype=np.int32)
+        _check(lib.kifmm_gpu_close_list(self._h, ptr.ctypes.data_as(_ip), ptr.size, idx.ctypes.data_as(_ip), n))
+        return ptr, idx[:n]
 
     def evaluate_phases(self, phases, copy=True):
         t = PhaseTimes()
-        pot = np.zeros(self.n_trg) if copy else None
+        out = self.output() if copy else None
+        pot = (np.zeros(self.n_trg) if out is None else out) if copy else None
         _check(lib.kifmm_gpu_evaluate_phases(self._h, phases, _dptr(pot) if copy else None, C.byref(t)))
-        return pot, t.as_dict()
+        return (pot.copy() if copy and out is not None else pot), t.as_dict()
 
     def tuning(self):
         t = GpuTuning()

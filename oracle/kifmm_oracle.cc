@@ -11,11 +11,11 @@
  *   target binning ........ fmm/input.cc:386-460
  *   U/V/W/X lists ......... fmm/input.cc:303-381 (sorted, SRCNODE-filtered)
  *   gather ................ fmm/evaluate.cc:13-96
- *   surfaces / grid ....... fmm/trans.cc:73-145
- *   operators + pinv ...... fmm/trans.cc:203-326, fmm/vecmatop.cc:34-78, fmm/svdrep.cc:10-66
+ *   surfaces / grid ....... fmm/trans.cc:13-88
+ *   operators + pinv ...... fmm/trans.cc:143-264, fmm/vecmatop.cc:34-78, fmm/svdrep.cc:10-66
  *                           (the SVD is a one-sided Jacobi written here; MKL DGESVD is absent)
  *   FFT (r2c/c2r) ......... FFTW layout [z][y][x/2+1], unnormalised inverse
- *                           (fmm/trans.cc:331-476); separable DFT written here
+ *                           (fmm/trans.cc:271-416); separable DFT written here
  *   evaluation ............ fmm/omp_sse_block.cc:60-887, fmm/evaluate--basic.cc:30-87
  *   validate .............. fmm/driver.cc:141-206
  *
@@ -79,7 +79,7 @@ void node_center(const Node& n, double c[3]) {
 }
 double node_radius(const Node& n) { return 1.0 / (double)(1 << n.depth); }
 
-/* fmm/trans.cc:91-120 */
+/* fmm/trans.cc:31-62 */
 Surf sampos(int np, double R) {
   Surf s;
   s.n = np * np * np - (np - 2) * (np - 2) * (np - 2);
@@ -98,7 +98,7 @@ Surf sampos(int np, double R) {
   return s;
 }
 
-/* fmm/trans.cc:124-145 */
+/* fmm/trans.cc:64-88 */
 Surf regpos(int np, double R) {
   Surf s;
   const int N = 2 * np;
@@ -118,7 +118,7 @@ Surf regpos(int np, double R) {
   return s;
 }
 
-/* grid offsets of the surface samples (fmm/trans.cc:395-417) */
+/* grid offsets of the surface samples (fmm/trans.cc:335-355,394-416) */
 std::vector<int> sample_grid_offsets(int np) {
   std::vector<int> off;
   const int N = 2 * np;
@@ -215,7 +215,7 @@ void kernel_mat(int sn, int tn, const double* x1, const double* x2, const double
   }
 }
 
-/* y (+)= A x, A column-major [m x n] (MKL dgemv 'N' as used in fmm/vecmatop.cc:18-25) */
+/* y (+)= A x, A column-major [m x n] (MKL dgemv 'N' as used in fmm/vecmatop.cc:15-22) */
 void gemv(int m, int n, const double* A, const double* x, double* y, bool accumulate) {
   std::vector<double> t(m, 0.0);
   for (int j = 0; j < n; j++) {
@@ -619,7 +619,7 @@ void build_trees(oracle_problem* p) {
 
 void setup_surfaces(oracle_problem* p) {
   const int np = p->np;
-  /* trans_setup, fmm/trans.cc:73-86 */
+  /* trans_setup, fmm/trans.cc:13-26 */
   p->SP[UE] = sampos(np, 1.0);
   p->SP[UC] = sampos(np + 2, 3.0);
   p->SP[DE] = sampos(np, 3.0);
@@ -738,14 +738,14 @@ void build_ops(oracle_problem* p, int mode) {
   const Surf& uc = p->SP[UC];
   const Surf& de = p->SP[DE];
   const Surf& dc = p->SP[DC];
-  const double eps = std::pow(0.1, p->np + 1); /* alt(), fmm/trans.cc:180-184 */
+  const double eps = std::pow(0.1, p->np + 1); /* alt(), fmm/trans.cc:120-124 */
   std::vector<double> K((size_t)ne * nuc);
-  /* compute_UC2UE_mat, fmm/trans.cc:203-229 */
+  /* compute_UC2UE_mat, fmm/trans.cc:143-168 */
   kernel_mat(ne, nuc, ue.x.data(), ue.y.data(), ue.z.data(), uc.x.data(), uc.y.data(), uc.z.data(),
              K.data(), mode);
   p->uc2ue.assign((size_t)ne * nuc, 0.0);
   pinv(K.data(), nuc, ne, eps, p->uc2ue.data());
-  /* compute_UE2UC_mat, fmm/trans.cc:234-263 */
+  /* compute_UE2UC_mat, fmm/trans.cc:174-203 */
   p->ue2uc.assign((size_t)8 * nuc * ne, 0.0);
   std::vector<double> cx(nuc), cy(nuc), cz(nuc), dx(ne), dy(ne), dz(ne);
   for (int o = 0; o < 8; o++) {
@@ -759,13 +759,13 @@ void build_ops(oracle_problem* p, int mode) {
     kernel_mat(ne, nuc, dx.data(), dy.data(), dz.data(), cx.data(), cy.data(), cz.data(),
                &p->ue2uc[(size_t)o * nuc * ne], mode);
   }
-  /* compute_DC2DE_mat, fmm/trans.cc:268-293 */
+  /* compute_DC2DE_mat, fmm/trans.cc:208-233 */
   std::vector<double> K2((size_t)ne * ne);
   kernel_mat(ne, ne, de.x.data(), de.y.data(), de.z.data(), dc.x.data(), dc.y.data(), dc.z.data(),
              K2.data(), mode);
   p->dc2de.assign((size_t)ne * ne, 0.0);
   pinv(K2.data(), ne, ne, eps, p->dc2de.data());
-  /* compute_DE2DC_mat, fmm/trans.cc:297-326 */
+  /* compute_DE2DC_mat, fmm/trans.cc:237-264 */
   p->de2dc.assign((size_t)8 * ne * ne, 0.0);
   std::vector<double> ex(ne), ey(ne), ez(ne);
   for (int o = 0; o < 8; o++) {
@@ -874,7 +874,7 @@ void fft_src(oracle_problem* p, std::vector<std::complex<double>>& eden, const D
     for (long i = 0; i < (long)nv.size(); i++) {
       if (!(nv[i].tag & SRCNODE)) continue;
       std::fill(reg.begin(), reg.end(), 0.0);
-      const double s = std::ldexp(1.0, nv[i].depth); /* fmm/trans.cc:344-355 */
+      const double s = std::ldexp(1.0, nv[i].depth); /* fmm/trans.cc:308-333 */
       for (int j = 0; j < ne; j++) reg[off[j]] = p->uden[(size_t)i * ne + j] * s;
       rfft3(F, reg.data(), &eden[(size_t)i * eff]);
     }
@@ -905,7 +905,7 @@ void vlist_pass(oracle_problem* p, int mode, int nthreads) {
   fft_src(p, eden, F);
   translation_spectra(np, mode, F, T);
   std::vector<std::complex<double>> eval((size_t)nv.size() * eff, 0.0);
-  /* work_division_V, fmm/evaluate.cc:313-345 */
+  /* work_division_V, fmm/evaluate.cc:143-176 */
   const int nn = (int)nv.size();
   std::vector<int> lo(nn + 1, 0), Ti(nthreads + 1, 0);
   for (int i = 0; i < nn; i++) lo[i + 1] = lo[i] + (int)nv[i].V.size();
@@ -1087,7 +1087,7 @@ void down_pass(oracle_problem* p, int mode) {
   }
 }
 
-/* ulist_calc__cpu, fmm/omp_sse_block.cc:236-261 with work_division_U (fmm/evaluate.cc:270-309) */
+/* ulist_calc__cpu, fmm/omp_sse_block.cc:236-261 with work_division_U (fmm/evaluate.cc:100-141) */
 void ulist_pass(oracle_problem* p, int mode, int nthreads) {
   const std::vector<Node>& nv = p->nodes;
   const int nn = (int)nv.size();

@@ -53,7 +53,7 @@ int oracle_points(const oracle_problem* p, double* sx, double* sy, double* sz,
 /* leaf-ordered (gathered) point index maps: src_perm[k] = original index */
 int oracle_perms(const oracle_problem* p, int* src_perm, int* trg_perm);
 
-/* Operators (fmm/trans.cc:203-326).  Sizes: n_e = np^3-(np-2)^3,
+/* Operators (fmm/trans.cc:143-264).  Sizes: n_e = np^3-(np-2)^3,
  * n_uc = (np+2)^3-np^3.  Column-major:
  *   uc2ue [n_e x n_uc], ue2uc 8 x [n_uc x n_e], dc2de [n_e x n_e],
  *   de2dc 8 x [n_e x n_e]; octant index = z + 2y + 4x. */

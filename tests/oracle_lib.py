@@ -231,7 +231,7 @@ def set_reverse_order(on):
 
 # Parity bound per NP.  NP <= 4: 1e-12.  NP = 6: the UC2UE / DC2DE
 # pseudo-inverses keep singular values down to 1e-7 of the largest
-# (cutoff 0.1^(NP+1), fmm/trans.cc:180-184), so rounding-level changes in
+# (cutoff 0.1^(NP+1), fmm/trans.cc:120-124), so rounding-level changes in
 # summation order move the potentials by ~2e-11 relative; the oracle
 # against itself with only its source sums reversed differs by 1.6e-11 to
 # 2.0e-11 (test_oracle.py::test_summation_order_floor).  The bound is 3x

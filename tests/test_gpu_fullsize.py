@@ -1,8 +1,11 @@
-"""GPU path at the benchmark sizes, through properties that do not need the
-CPU oracle (too slow there): linearity in the densities, the sampled error
-against the FP64 direct sum at NV = 1000 (direct sums on the GPU, same
-sample as validate, fmm/driver.cc:141-206), graph replay = eager, and the
-mixed path's error against the FP64 path's on the same tree."""
+"""GPU path at the benchmark sizes: the FP64 potentials against the oracle
+(the restated reference CPU path, exact kernel, same operators) on the
+4,194,304-point NP = 6 input and the 1,048,576-point ellipsoid with the
+autotuned shapes in one HIP graph, at the parity bound; plus properties:
+linearity in the densities, the sampled error against the FP64 direct sum
+at NV = 1000 (direct sums on the GPU, same sample as validate,
+fmm/driver.cc:141-206), graph replay = eager, and the mixed path's error
+against the FP64 path's on the same tree."""
 import numpy as np
 import pytest
 
@@ -14,6 +17,53 @@ pytestmark = pytest.mark.gpu
 
 def rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def oracle_potentials(prob, n, dist, q, np_, seed, c=0):
+    """The oracle on the same seeded input with the library's operators."""
+    o = prob.operators()
+    orc = O.Oracle(n, dist, q, np_, seed, c)
+    orc.set_operators(*[np.ctypeslib.as_array(p, shape=(k,)).copy() for p, k in
+                        ((o.uc2ue, o.n_e * o.n_uc), (o.ue2uc, 8 * o.n_uc * o.n_e), (o.dc2de, o.n_e ** 2),
+                         (o.de2dc, 8 * o.n_e ** 2))])
+    ref, t = orc.evaluate(O.KERNEL_EXACT)
+    print(f"  oracle evaluated in {t['total']:.1f} s", flush=True)
+    return ref
+
+
+def test_uniform_4m_np6_matches_oracle():
+    """4,194,304 uniform points, seed 1, NP = 6, 512 per box: the plain FP64
+    GPU path against the oracle on all 4,194,304 potentials."""
+    print("case 4M uniform NP6 q512 seed1 vs oracle", flush=True)
+    prob = K.Problem.create(4194304, "uniform", 512, np_=6, seed=1)
+    ev = K.GpuEvaluator(prob)
+    got = prob.scatter(ev.evaluate()[0])
+    del ev
+    ref = oracle_potentials(prob, 4194304, "uniform", 512, 6, 1)
+    r = rel(got, ref)
+    print(f"  GPU vs oracle {r:.3e} (bound {O.PARITY_RTOL[6]:g})", flush=True)
+    assert r < O.PARITY_RTOL[6]
+
+
+def test_ellipsoid_1m_np4_tuned_graph_matches_oracle():
+    """1,048,576 pole-clustered ellipsoid points, seed 2, NP = 4, 128 per
+    box, FP64 with the autotuned U-list form, Hadamard block and stream
+    layout, replayed as one HIP graph: against the oracle on every potential."""
+    print("case 1M ellipsoid NP4 q128 seed2 tuned graph vs oracle", flush=True)
+    prob = K.Problem.create(1048576, "ellipsoid", 128, np_=4, seed=2)
+    tuner = K.GpuEvaluator(prob)
+    best, _ = tuner.autotune(reps=2)
+    del tuner
+    ev = K.GpuEvaluator(prob, graph=True)
+    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"], best["overlap"])
+    got = prob.scatter(ev.evaluate()[0])
+    again = prob.scatter(ev.evaluate()[0])
+    assert np.array_equal(got, again)
+    del ev
+    ref = oracle_potentials(prob, 1048576, "ellipsoid", 128, 4, 2)
+    r = rel(got, ref)
+    print(f"  tuned {best}: GPU vs oracle {r:.3e} (bound {O.PARITY_RTOL[4]:g})", flush=True)
+    assert r < O.PARITY_RTOL[4]
 
 
 def test_uniform_4m_np6_linearity_and_error():

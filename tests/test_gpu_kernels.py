@@ -3,7 +3,7 @@
 Pins the pieces the end-to-end parity test composes: the MFMA operand
 layout, the FP64 GEMM, the reciprocal square root, the forward/inverse
 pruned DFTs (FFTW r2c/c2r conventions: [z][y][x/2+1], unnormalised
-inverse, fmm/trans.cc:331-476) and the 316 translation spectra
+inverse, fmm/trans.cc:271-416) and the 316 translation spectra
 (fmm/omp_sse_block.cc:432-478).
 """
 import numpy as np
@@ -96,7 +96,7 @@ def test_rsqrt_accuracy():
 
 
 def _grid_from_surface(np_, vals):
-    """scatter surface samples onto the (2np)^3 grid at offset np/2 (fmm/trans.cc:395-417)"""
+    """scatter surface samples onto the (2np)^3 grid at offset np/2 (fmm/trans.cc:335-355,394-416)"""
     N = 2 * np_
     g = np.zeros((N, N, N))
     cnt = 0

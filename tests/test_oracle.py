@@ -93,7 +93,7 @@ def K_pinv(M, eps):
 
 def test_product_operators_match_definitions():
     """UE2UC / DE2DC are plain kernel matrices: check against numpy from the
-    surfaces and the shifts of fmm/trans.cc:234-263 and 297-326."""
+    surfaces and the shifts of fmm/trans.cc:174-203 and 237-264."""
     np_ = 4
     p = K.Problem.create(1000, "uniform", 64, np_=np_, seed=1)
     ops = p.operator_arrays()
@@ -173,6 +173,52 @@ def test_oracle_sampled_error_np6():
     pot, _ = o.evaluate(O.KERNEL_EXACT)
     err, _, _ = o.validate(pot, 1000)
     assert err < 1e-6
+
+
+@pytest.mark.parametrize("dist,n,q,np_,c,bound", [("gaussmix", 6000, 16, 4, 0, 1e-4), ("ellipsoid", 8000, 8, 4, 1, 1e-4),
+                                                  ("gaussmix", 5000, 24, 6, 1, 1e-6)])
+def test_oracle_with_w_and_x_lists_against_the_direct_sum(dist, n, q, np_, c, bound):
+    """Non-uniform trees, whose W and X lists are not empty: the oracle's
+    potentials at every target against the FP64 direct sum over all sources
+    (relative L2 over all targets, not a sample).  The uniform inputs of the
+    other error tests have empty W and X lists (fmm/omp_sse_block.cc:663-756
+    is not exercised there)."""
+    o = O.Oracle(n, dist, q, np_, 15, c)
+    t = o.tree()
+    nw, nx = int(t["w_ptr"][-1]), int(t["x_ptr"][-1])
+    assert nw > 100 and nx > 100, (nw, nx)
+    o.build_operators(O.KERNEL_EXACT)
+    pot, _ = o.evaluate(O.KERNEL_EXACT)
+    p = o.points()
+    ref = O.direct((p["tx"], p["ty"], p["tz"]), (p["sx"], p["sy"], p["sz"]), p["den"])
+    err = np.linalg.norm(pot - ref) / np.linalg.norm(ref)
+    # the W terms carry a share of the potential far above that error: for a
+    # few W-bearing target leaves, the direct sum over the sources below
+    # their W boxes against the potential there
+    sp, tp = o.perms()  # gathered index -> input index
+    leaf = np.flatnonzero((t["child"] == -1) & (np.diff(t["w_ptr"]) > 0) & (t["trg_num"] > 0))
+    share = []
+    for i in leaf[:: max(1, len(leaf) // 20)]:
+        trg = tp[t["trg_beg"][i]:t["trg_beg"][i] + t["trg_num"][i]]
+        src = sp[np.concatenate([np.arange(t["src_beg"][v], t["src_beg"][v] + t["src_num"][v])
+                                 for v in _leaves_below(t, t["w_idx"][t["w_ptr"][i]:t["w_ptr"][i + 1]])])]
+        w_part = O.direct((p["tx"][trg], p["ty"][trg], p["tz"][trg]), (p["sx"][src], p["sy"][src], p["sz"][src]),
+                          p["den"][src])
+        share.append(np.linalg.norm(w_part) / np.linalg.norm(ref[trg]))
+    print(f"  {dist} NP={np_}: W {nw}, X {nx} entries; error vs direct sum {err:.3e}; "
+          f"W share at sampled leaves {np.median(share):.2e}")
+    assert err < bound and np.median(share) > 100 * err
+
+
+def _leaves_below(t, nodes):
+    out, stack = [], list(nodes)
+    while stack:
+        v = stack.pop()
+        if t["child"][v] == -1:
+            out.append(v)
+        else:
+            stack.extend(range(t["child"][v], t["child"][v] + 8))
+    return sorted(out)
 
 
 @pytest.mark.parametrize("np_,case", [(4, (65536, "uniform", 64, 4, 0, 1)), (6, (32768, "uniform", 64, 6, 1, 0))])
