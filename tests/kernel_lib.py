@@ -1,11 +1,15 @@
-"""ctypes signatures of the kernel-level test entry points (include/kifmm_kernels.h)."""
+"""ctypes signatures of the kernel-level test entry points (include/kifmm_kernels.h),
+built into the test-only build/libkifmm_ktest.so (not the product library)."""
 import ctypes as C
 
 import numpy as np
 
 import kifmm_amd as K
 
-lib = K.lib
+KTEST = K.HERE / "build" / "libkifmm_ktest.so"
+if not KTEST.exists():
+    raise ImportError(f"kernel test library not built: {KTEST} (run __graft_entry__.build())")
+lib = C.CDLL(str(KTEST))
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 for name, args in (

@@ -20,12 +20,23 @@ def declared(header):
     return sorted(set(re.findall(r"\b((?:kifmm|oracle)_\w+)\s*\(", text)))
 
 
-@pytest.mark.parametrize("header", ["include/kifmm_host.h", "include/kifmm_hip.h", "include/kifmm_kernels.h"])
+@pytest.mark.parametrize("header", ["include/kifmm_host.h", "include/kifmm_hip.h", "include/kifmm_mlp.h"])
 def test_library_exports_every_declared_symbol(header):
     names = declared(ROOT / header)
     assert names
     missing = [n for n in names if not hasattr(K.lib, n)]
     assert not missing, missing
+
+
+def test_kernel_probes_live_in_the_test_library_only():
+    """include/kifmm_kernels.h is exported by build/libkifmm_ktest.so, and
+    none of its probes ships in the product library."""
+    import kernel_lib as KL
+
+    names = declared(ROOT / "include/kifmm_kernels.h")
+    assert names
+    assert not [n for n in names if not hasattr(KL.lib, n)]
+    assert not [n for n in names if hasattr(K.lib, n)]
 
 
 def test_oracle_exports_every_declared_symbol():
