@@ -34,9 +34,11 @@ enum {
 };
 
 enum {
-  KIFMM_GPU_GRAPH = 1, /* capture the whole evaluation in one HIP graph at the first full
-                          evaluate, replay it afterwards (phase times then report only
-                          total and copy) */
+  KIFMM_GPU_GRAPH = 1, /* capture the far-field launch sequence (fills, up, V, W, X, down)
+                          in one HIP graph at the first full evaluate and replay it
+                          afterwards; the U-list chunks and their copies back, which run
+                          beside one another on three streams, stay outside the graph
+                          (phase times then report the U-list, the copy and the total) */
   KIFMM_GPU_NO_CLOSE_LIST = 2, /* do not list the U-list pairs closer than 1/8 of the target
                                   leaf's half-width at create time: the mixed path then tests
                                   for them inside its loop, the FP64 path forms r^2 from
@@ -59,15 +61,21 @@ enum {
 enum {
   KIFMM_P2P_STATIC = 0,     /* one block per U-list item of <= 128 targets (default) */
   KIFMM_P2P_PERSISTENT = 1, /* a few blocks per CU drawing the same items from per-XCD queues */
-  KIFMM_P2P_STATIC64 = 2    /* one block per item of <= 64 targets */
+  KIFMM_P2P_STATIC64 = 2,   /* one block per item of <= 64 targets */
+  KIFMM_P2P_SPLIT = 3       /* mixed path only: items of <= 128 targets, up to 4 groups of 16
+                               targets per wave, each source tile split among the waves that
+                               share a set of groups (pays on small leaves) */
 };
 typedef struct kifmm_gpu_tuning {
-  int p2p_form;      /* KIFMM_P2P_*; the FP64 U-list only (the mixed path has one form) */
+  int p2p_form;      /* KIFMM_P2P_*: STATIC, PERSISTENT or STATIC64 on the FP64 path;
+                        STATIC or SPLIT on the mixed path */
   int hadamard_fb;   /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
   int hadamard_form; /* KIFMM_HAD_*: V-list Hadamard on MFMA (default) or on the VALU */
-  int overlap;       /* stream layout: 0 the U-list last, in chunks whose copies back
-                        overlap the later chunks; 1 the U-list on the handle's second stream
-                        alongside everything else (initially 1 with KIFMM_GPU_OVERLAP) */
+  int overlap;       /* stream layout of the U-list's copy-out chunks: 0 all of them last,
+                        after the far field, their copies back overlapping the later
+                        chunks; 1 all of them on the handle's second stream from the
+                        start, alongside the far field (initially 1 with KIFMM_GPU_OVERLAP);
+                        2, 3 the first one or two chunks there, the rest last */
   int reserved[4];
 } kifmm_gpu_tuning;
 enum {
@@ -99,8 +107,7 @@ typedef struct kifmm_phase_times {
   double copy;      /* device -> host potentials: the part left after the last
                        kernel (the copies of earlier U-list chunks overlap the later
                        chunks), plus the host pass into a caller buffer that is not
-                       kifmm_gpu_output()'s; 0 in graph mode, whose copies are inside
-                       the graph (total covers them) */
+                       kifmm_gpu_output()'s */
   double total;     /* first event to last event (+ that host pass) */
   double p2p_kernel;/* the U-list kernel alone            */
   double reserved[3];
@@ -193,20 +200,22 @@ int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long
 
 /* Set or read the launch shapes and the stream layout.  The U-list forms,
  * the Hadamard block sizes and the layouts give bit-identical potentials;
- * the two Hadamard forms sum in different orders (rounding-level
- * differences).  Setting drops a captured graph (the next full evaluate
- * captures again).  EINVAL for an unknown value, EUNSUP for a U-list form
- * other than STATIC on the mixed path. */
+ * the two Hadamard forms sum in different orders, and so do the two
+ * mixed U-list forms (rounding-level differences).  Setting drops a
+ * captured graph (the next full evaluate captures again).  EINVAL for an
+ * unknown value, EUNSUP for a U-list form the handle's path does not have
+ * (PERSISTENT / STATIC64 on the mixed path, SPLIT on the FP64 path or
+ * without a close-pair list). */
 int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t);
 int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
-/* Time every U-list form (FP64 path) on the U phase, every Hadamard form
- * and block size on the V phase, then both stream layouts on whole (eager)
- * evaluations of this handle's own input, reps timed runs each after one
- * warm-up, keep the fastest of each (minimum over runs) and set it.  best
- * (may be NULL) receives the choice; ms (may be NULL, n_ms entries, up to
- * 11) receives the times in the order STATIC, PERSISTENT, STATIC64, VALU
- * fb 4, 8, 16, MFMA fb 4, 8, 16, one stream, overlap; -1 for a candidate
- * not timed. */
+/* Time every U-list form of the handle's path on the U phase, every
+ * Hadamard form and block size on the V phase, then both stream layouts on
+ * whole (eager) evaluations with the copy back of this handle's own input,
+ * reps timed runs each after one warm-up, keep the fastest of each (minimum
+ * over runs) and set it.  best (may be NULL) receives the choice; ms (may
+ * be NULL, n_ms entries, up to 14) receives the times in the order STATIC,
+ * PERSISTENT, STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16, layout 0,
+ * layout 1, SPLIT, layout 2, layout 3; -1 for a candidate not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */
@@ -217,6 +226,18 @@ int kifmm_gpu_available(void);
  * compensated summation.  Replaces the O(NV N) host loop of validate
  * (fmm/driver.cc:141-206). */
 int kifmm_gpu_direct(kifmm_gpu* g, int nt, const double* tx, const double* ty, const double* tz, double* out);
+
+/* kifmm_gpu_direct without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The targets and sums
+ * are staged in a file-scope device array of the library holding at most
+ * KIFMM_DIRECT_NOALLOC_MAX points (512 KiB per device, kept until the
+ * process ends); larger nt is rejected with KIFMM_EINVAL.  Every call on
+ * one device shares that array: calls that overlap on one device (other
+ * threads, other handles) race.  Blocks until the sums are in out. */
+#define KIFMM_DIRECT_NOALLOC_MAX 16384
+int kifmm_gpu_direct_noalloc(kifmm_gpu* g, int nt, const double* tx, const double* ty, const double* tz,
+                             double* out);
 
 /* kifmm_problem_validate with the direct sums on the GPU: the same sample
  * (the problem's generator stream continued) and the same relative L2

@@ -700,10 +700,11 @@ __device__ __forceinline__ v8bf gram_a_block(double x, double y, double z, int k
   return out;
 }
 
+/* chunks [ch_lo, ch_hi) of 16 sources of the staged tile (an even count) */
 template <int G>
 __device__ __forceinline__ void mixed_tau_chunks(const v8bf (*sB)[P2P_TILE], const float* __restrict__ sW,
-                                                 int nchunks, const v8bf* a, const v4f* c, double (*acc)[4],
-                                                 int lane) {
+                                                 int ch_lo, int ch_hi, const v8bf* a, const v4f* c,
+                                                 double (*acc)[4], int lane) {
   const int j = lane & 15;
   const v8bf* __restrict__ col = sB[lane >> 4];
   float part[G][4];
@@ -711,7 +712,7 @@ __device__ __forceinline__ void mixed_tau_chunks(const v8bf (*sB)[P2P_TILE], con
   for (int q = 0; q < G; q++)
 #pragma unroll
     for (int e = 0; e < 4; e++) part[q][e] = 0.0f;
-  for (int ch = 0; ch < nchunks; ch += 2) { /* nchunks is even */
+  for (int ch = ch_lo; ch < ch_hi; ch += 2) {
 #pragma unroll
     for (int u = 0; u < 2; u++) {
       const int s = (ch + u) * 16 + j;
@@ -802,9 +803,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     }
     __syncthreads();
     const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
-    if (wave + 8 < ngroups) mixed_tau_chunks<3>(sB, sW, ns >> 4, a, c, acc, lane);
-    else if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, ns >> 4, a, c, acc, lane);
-    else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, ns >> 4, a, c, acc, lane);
+    if (wave + 8 < ngroups) mixed_tau_chunks<3>(sB, sW, 0, ns >> 4, a, c, acc, lane);
+    else if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, 0, ns >> 4, a, c, acc, lane);
+    else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, 0, ns >> 4, a, c, acc, lane);
     fill = next;
   }
   /* FP64 close-pair terms of the wave's target groups.  A group's pairs are
@@ -869,6 +870,153 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         }
       }
     }
+  }
+}
+
+/* Mixed U-list, source-split form (items of <= P2P_MAXT targets).  The
+ * arithmetic is the tau kernel's (bf16 Gram MFMA distances, clamped FP32
+ * rsq, listed FP64 close-pair terms); the work is divided differently.  The
+ * item's 16-target groups form sets of up to MIXS_G = 4 groups (one set for
+ * <= 64 targets, two for 65-128), and the 4 / sets waves of a set split
+ * every staged source tile between them (contiguous chunk ranges), each
+ * running all of the set's groups.  So every wave holds up to 4 groups and
+ * reads each B operand once for up to 4 MFMAs, where the tau kernel gives a
+ * wave of a 64-target leaf a single group.  The partial sums of a set's
+ * waves are added per target in wave order through LDS, then the close-
+ * pair terms.  Sums differ from the tau kernel's at rounding level (other
+ * partial sums), and are bit-identical from run to run. */
+constexpr int MIXS_G = 4;
+template <bool MASK>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_p2p_ulist_mixed_split(
+    const int4* __restrict__ items, int n_items, int per_xcd, const int* __restrict__ rng_ptr,
+    const int2* __restrict__ rng, const double4* __restrict__ geo, const double4* __restrict__ src4,
+    const double4* __restrict__ trg4, const int* __restrict__ cptr, const int* __restrict__ cidx,
+    double* __restrict__ pot, int accum) {
+  __shared__ v8bf sB[4][P2P_TILE];
+  __shared__ float sW[P2P_TILE];
+  __shared__ double4 sT[P2P_MAXT];
+  const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  if (item >= n_items) return; /* whole block, before any barrier */
+  const int4 it = items[item];
+  const double4 g = geo[it.w];
+  const double sc = (1.0 / MIX_CLOSE) / g.w; /* 1 / (tau h), a power of two */
+  const int tid = threadIdx.x, lane = tid & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+  const int ngroups = (it.z + 15) >> 4; /* 1 .. 8 */
+  const int nsets = ngroups <= MIXS_G ? 1 : 2;
+  const int wps = 4 / nsets; /* waves per set: the source split */
+  const int set = wave / wps, part = wave % wps;
+  const int g0 = MIXS_G * set, gn = min(MIXS_G, ngroups - g0); /* gn >= 1 for every set */
+  for (int k = tid; k < ngroups * 16; k += 256) {
+    const double4 q = trg4[it.y + min(k, it.z - 1)];
+    const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
+    sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
+  }
+  __syncthreads();
+  v8bf a[MIXS_G];
+  v4f c[MIXS_G];
+  double acc[MIXS_G][4];
+#pragma unroll
+  for (int q = 0; q < MIXS_G; q++) {
+    const int grp = min(g0 + q, ngroups - 1);
+    const double4 t = sT[16 * grp + (lane & 15)];
+    const int k4 = lane >> 4;
+    a[q] = gram_a_block(t.x, t.y, t.z, k4);
+#pragma unroll
+    for (int e = 0; e < 4; e++) {
+      c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
+      acc[q][e] = 0.0;
+    }
+  }
+  RangeWalk walk{rng_ptr[it.x], 0, rng_ptr[it.x + 1]};
+  double4 pre[P2P_PER];
+  int fill = walk_tile(walk, rng, src4, tid, pre);
+  while (fill > 0) {
+    __syncthreads(); /* the previous tile's readers are done */
+#pragma unroll
+    for (int m = 0; m < P2P_PER; m++) {
+      const int k = tid + 256 * m;
+      if (k < fill) {
+        const double4 q = pre[m];
+        const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
+        gram_b_rows(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)), sB, k);
+        sW[k] = (float)q.w;
+      }
+    }
+    const int ns = (fill + 31) & ~31; /* whole chunk pairs */
+    for (int k = fill + tid; k < ns; k += 256) {
+      gram_b_rows(MIX_FAR, MIX_FAR, MIX_FAR, 3.0 * MIX_FAR * MIX_FAR, sB, k);
+      sW[k] = 0.0f;
+    }
+    __syncthreads();
+    const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
+    const int npairs = ns >> 5; /* chunk pairs of the tile, split among the set's waves */
+    const int lo = 2 * (npairs * part / wps), hi = 2 * (npairs * (part + 1) / wps);
+    if (gn == 4) mixed_tau_chunks<4>(sB, sW, lo, hi, a, c, acc, lane);
+    else if (gn == 3) mixed_tau_chunks<3>(sB, sW, lo, hi, a, c, acc, lane);
+    else if (gn == 2) mixed_tau_chunks<2>(sB, sW, lo, hi, a, c, acc, lane);
+    else mixed_tau_chunks<1>(sB, sW, lo, hi, a, c, acc, lane);
+    fill = next;
+  }
+  /* the tile's LDS now holds the per-wave partial sums red[wave][target] and
+   * the close-pair sums sC[target] */
+  static_assert(sizeof(sB) >= (4 + 1) * P2P_MAXT * sizeof(double), "partial sums in the tile");
+  __syncthreads();
+  double* const red = reinterpret_cast<double*>(&sB[0][0]);
+  double* const sC = red + 4 * P2P_MAXT;
+#pragma unroll
+  for (int q = 0; q < MIXS_G; q++) {
+    if (q < gn) {
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        double v = acc[q][e];
+        v += __shfl_xor(v, 8, 64);
+        v += __shfl_xor(v, 4, 64);
+        v += __shfl_xor(v, 2, 64);
+        v += __shfl_xor(v, 1, 64);
+        const int t = 16 * (g0 + q) + 4 * (lane >> 4) + e;
+        if ((lane & 15) == 0) red[part * P2P_MAXT + t] = v;
+      }
+    }
+  }
+  /* FP64 close-pair terms, group grp by wave grp % 4 (as in the tau kernel) */
+  for (int grp = wave; grp < ngroups; grp += 4) {
+    const int t0 = 16 * grp, nt = min(16, it.z - t0), tg0 = it.y + t0;
+    if (lane < 16) sC[t0 + lane] = 0.0;
+    const int kbeg = cptr[tg0], kend = cptr[tg0 + nt];
+    const int ej = cptr[tg0 + min(lane & 15, nt - 1) + 1];
+    for (int kb = kbeg; kb < kend; kb += 64) {
+      const int k = kb + lane;
+      const bool live = k < kend;
+      int tl = 0;
+#pragma unroll
+      for (int j = 0; j < 16; j++) tl += __builtin_amdgcn_readlane(ej, j) <= k;
+      tl = live ? tl : 16;
+      double v = 0.0;
+      if (live) {
+        const double4 sp = src4[cidx[k]];
+        const double4 tp = sT[t0 + tl];
+        const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
+        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        v = sp.w * (inv_r<MASK>(r2) - 1.0);
+      }
+#pragma unroll
+      for (int off = 1; off < 64; off <<= 1) {
+        const double u = __shfl_up(v, off, 64);
+        const int ut = __shfl_up(tl, off, 64);
+        if (lane >= off && ut == tl) v += u;
+      }
+      const int nx = __shfl_down(tl, 1, 64);
+      if (live && (lane == 63 || nx != tl)) sC[t0 + tl] += v;
+    }
+  }
+  __syncthreads();
+  const double scale = OOFP_R * sc;
+  for (int t = tid; t < it.z; t += 256) {
+    double v = red[t]; /* the target's set: its waves' partial sums in wave order */
+    for (int p_ = 1; p_ < wps; p_++) v += red[p_ * P2P_MAXT + t];
+    const double u = scale * (v + sC[t]);
+    pot[it.y + t] = accum ? u + pot[it.y + t] : u;
   }
 }
 
@@ -1216,7 +1364,11 @@ __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items
  * on gfx950 by test_mfma_f64_operand_layout): A[i][k] in lane i + 16k,
  * B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4.
  * With ws != nullptr the unscaled partial product of K-slice blockIdx.z
- * goes to ws[(z * ncols + j) * ldw + row] for k_gemm_reduce. */
+ * goes to ws[(z * ncols + j) * ldw + row]; the last of the nsplit blocks of
+ * an output tile to finish (a device-scope counter per tile, tile_cnt,
+ * which that block resets to 0) then adds the partials in slice order and
+ * writes the tile, so a split GEMM is one launch and its sums do not depend
+ * on which block finishes last. */
 constexpr int GEMM_KC = 16;
 template <int RT, int CT>
 __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* __restrict__ A, int lda,
@@ -1225,7 +1377,8 @@ __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* _
                                                    const int* __restrict__ sexp,
                                                    const double* __restrict__ in,
                                                    double* __restrict__ out, int accumulate,
-                                                   int ksplit, double* __restrict__ ws, int ldw) {
+                                                   int ksplit, double* __restrict__ ws, int ldw,
+                                                   int* __restrict__ tile_cnt) {
   constexpr int ROWS = 16 * RT, NC = 64 * CT;
   /* row strides chosen so the 4 k-rows a wave reads hit disjoint bank halves */
   constexpr int LA = ROWS + (((16 - ROWS) % 32) + 32) % 32;
@@ -1302,22 +1455,196 @@ __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* _
       }
     }
   }
-}
-
-/* Sums the nsplit partial products of a split-K GEMM in split order and
- * applies scale / accumulate as k_gemm_cols does. */
-static __global__ __launch_bounds__(256) void k_gemm_reduce(int M, int ncols, int nsplit, const double* __restrict__ ws,
-                                                     int ldw, const int* __restrict__ out_off,
-                                                     const int* __restrict__ sexp, double* __restrict__ out,
-                                                     int accumulate) {
-  const size_t n = (size_t)ncols * M;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const int col = (int)(e / M), row = (int)(e % M);
+  if (!ws) return;
+  /* split K: the last block of this output tile adds the partials */
+  __shared__ int last;
+  __threadfence(); /* this block's partials are visible device-wide */
+  __syncthreads();
+  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
+  if (tid == 0) last = atomicAdd(tile_cnt + tile, 1) == (int)gridDim.z - 1;
+  __syncthreads();
+  if (!last) return;
+  __threadfence(); /* and the other blocks' partials are visible here */
+  for (int e = tid; e < NC * ROWS; e += 256) {
+    const int col = col0 + e / ROWS, row = row0 + e % ROWS;
+    if (col >= ncols || row >= M) continue;
     double v = 0.0;
-    for (int z = 0; z < nsplit; z++) v += ws[((size_t)z * ncols + col) * ldw + row];
+    for (int z = 0; z < (int)gridDim.z; z++) v += ws[((size_t)z * ncols + col) * ldw + row];
     double* o = out + out_off[col] + row;
     const double t = ldexp(1.0, sexp[col]) * v;
     *o = accumulate ? *o + t : t;
+  }
+  if (tid == 0) tile_cnt[tile] = 0; /* ready for the next split launch on this stream */
+}
+
+/* One level of the downward pass with its two operator applies fused
+ * (down_eval__gpu_non_leaves, fmm/cuda.cu:1682-1801): a block owns up to 16
+ * parents at depth d - 1 (in_off[j] = their downward densities, out_off[j]
+ * = their first child's check values) and one child octant o = blockIdx.y.
+ *   phase 1  T = dchk[child] + 2^(d-1) DE2DC[o] dden[parent]   (L2L; written
+ *            back to dchk for every child)
+ *   phase 2  dden[child] = 2^-d DC2DE T                         (DC2DE, for the
+ *            children that hold targets)
+ * Both are v_mfma_f64_16x16x4f64 products over the 16 parent columns (the
+ * operand layout of k_gemm_cols), K walked in steps of 4 in order, with the
+ * parents' densities and T staged in LDS: one launch per level where the
+ * column GEMMs took two, and the check values never make a round trip
+ * through HBM between them.  Rows RT * 16 >= n_e; wave w takes row tiles
+ * w, w + 4, ... of each phase. */
+template <int RT>
+__global__ __launch_bounds__(256) void k_l2l_dc2de(int ne, int lde, const double* __restrict__ DE2DC, int lda_l2l,
+                                                   const double* __restrict__ DC2DE, int lda_e, int npar,
+                                                   const int* __restrict__ in_off, const int* __restrict__ out_off,
+                                                   const int* __restrict__ tag, int d, double* __restrict__ dchk,
+                                                   double* __restrict__ dden) {
+  constexpr int ROWS = 16 * RT;
+  constexpr int LDP = 160 + 1; /* >= lde for NP <= 6; odd: conflict-free column reads */
+  constexpr int LDT = ROWS + 1;
+  __shared__ double Dp[16][LDP];
+  __shared__ double T[16][LDT];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int o = blockIdx.y, j0 = blockIdx.x * 16;
+  const int jn = min(16, npar - j0);
+  for (int e = tid; e < 16 * lde; e += 256) {
+    const int j = e / lde, k = e % lde;
+    Dp[j][k] = j < jn ? dden[(size_t)in_off[j0 + j] + k] : 0.0;
+  }
+  __syncthreads();
+  const int jl = lane & 15, kl = lane >> 4;
+  const double s1 = ldexp(1.0, d - 1), s2 = ldexp(1.0, -d);
+  const bool live = jl < jn;
+  const size_t child_off = live ? (size_t)out_off[j0 + jl] + (size_t)o * lde : 0; /* this lane's column */
+  /* a wave's row tiles share each B value and keep independent
+   * accumulator chains in flight */
+  constexpr int TPW = (RT + 3) / 4;
+  v4d acc[TPW];
+#pragma unroll
+  for (int u = 0; u < TPW; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < lde; k0 += 4) {
+    const double b = Dp[jl][k0 + kl];
+#pragma unroll
+    for (int u = 0; u < TPW; u++) {
+      const int row_a = 16 * (wave + 4 * u) + (lane & 15);
+      const double a = row_a < ne ? DE2DC[(size_t)(k0 + kl) * lda_l2l + (size_t)o * lde + row_a] : 0.0;
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < TPW; u++) {
+    const int r = wave + 4 * u;
+    if (r >= RT) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 16 * r + kl + 4 * i;
+      double t = 0.0;
+      if (live && row < ne) {
+        t = dchk[child_off + row] + s1 * acc[u][i];
+        dchk[child_off + row] = t;
+      }
+      T[jl][row] = t;
+    }
+  }
+  __syncthreads();
+  const bool trg = live && (tag[child_off / lde] & 2) != 0; /* KIFMM_TRGNODE */
+#pragma unroll
+  for (int u = 0; u < TPW; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < lde; k0 += 4) {
+    const double b = T[jl][k0 + kl];
+#pragma unroll
+    for (int u = 0; u < TPW; u++) {
+      const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1); /* rows past RT tiles: unused */
+      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(DC2DE[(size_t)(k0 + kl) * lda_e + row_a], b, acc[u], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < TPW; u++) {
+    const int r = wave + 4 * u;
+    if (r >= RT) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 16 * r + kl + 4 * i;
+      if (trg && row < ne) dden[child_off + row] = s2 * acc[u][i];
+    }
+  }
+}
+
+/* One level of the upward pass with its two operator applies fused
+ * (up_eval__gpu_non_leaf + up_calc__gpu_reduction, fmm/cuda.cu:430-566): a
+ * block owns up to 16 parents at depth d (in_off[j] = their 8 children's
+ * upward densities, 8 * lde contiguous doubles; out_off[j] = their own).
+ *   phase 1  chk = 2^(d+1) UE2UC_stack (children's densities)   (UC check values)
+ *   phase 2  uden[parent] = 2^-d UC2UE chk
+ * v_mfma_f64_16x16x4f64 over the 16 parent columns, K in steps of 4 in
+ * order (the column GEMM's order, so its sums are bit-identical to two
+ * unsplit k_gemm_cols launches), the children's densities staged through LDS
+ * 64 k at a time and chk kept in LDS between the phases.  RT1 / RT2: row
+ * tiles of the two operators (lda_uc / 16, lda_e / 16). */
+constexpr int M2M_KC = 64;
+template <int RT1, int RT2>
+__global__ __launch_bounds__(256) void k_m2m(int ne, int lde, int lduc, const double* __restrict__ UE2UC, int lda_uc,
+                                             const double* __restrict__ UC2UE, int lda_e, int npar,
+                                             const int* __restrict__ in_off, const int* __restrict__ out_off, int d,
+                                             double* __restrict__ uden) {
+  constexpr int LDC = 16 * RT1 + 1; /* odd: conflict-free column reads */
+  __shared__ double Bs[16][M2M_KC + 1];
+  __shared__ double C[16][LDC];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int j0 = blockIdx.x * 16, jn = min(16, npar - j0);
+  const int jl = lane & 15, kl = lane >> 4;
+  const int K = 8 * lde;
+  constexpr int TPW1 = (RT1 + 3) / 4, TPW2 = (RT2 + 3) / 4;
+  v4d acc[TPW1];
+#pragma unroll
+  for (int u = 0; u < TPW1; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int kc = 0; kc < K; kc += M2M_KC) {
+    const int kn = min(M2M_KC, K - kc); /* a multiple of 4 (8 lde) */
+    __syncthreads(); /* the previous chunk's readers are done */
+    for (int e = tid; e < 16 * M2M_KC; e += 256) {
+      const int j = e / M2M_KC, k = e % M2M_KC;
+      Bs[j][k] = (j < jn && k < kn) ? uden[(size_t)in_off[j0 + j] + kc + k] : 0.0;
+    }
+    __syncthreads();
+    for (int k0 = 0; k0 < kn; k0 += 4) {
+      const double b = Bs[jl][k0 + kl];
+#pragma unroll
+      for (int u = 0; u < TPW1; u++) {
+        const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_uc - 1); /* rows past RT1 tiles: unused */
+        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(UE2UC[(size_t)(kc + k0 + kl) * lda_uc + row_a], b, acc[u], 0,
+                                                      0, 0);
+      }
+    }
+  }
+  const double s1 = ldexp(1.0, d + 1), s2 = ldexp(1.0, -d);
+#pragma unroll
+  for (int u = 0; u < TPW1; u++) {
+    const int r = wave + 4 * u;
+    if (r >= RT1) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) C[jl][16 * r + kl + 4 * i] = s1 * acc[u][i];
+  }
+  __syncthreads();
+  v4d acc2[TPW2];
+#pragma unroll
+  for (int u = 0; u < TPW2; u++) acc2[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+  for (int k0 = 0; k0 < lduc; k0 += 4) {
+    const double b = C[jl][k0 + kl];
+#pragma unroll
+    for (int u = 0; u < TPW2; u++) {
+      const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1);
+      acc2[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(UC2UE[(size_t)(k0 + kl) * lda_e + row_a], b, acc2[u], 0, 0, 0);
+    }
+  }
+  if (jl >= jn) return;
+  double* const out = uden + out_off[j0 + jl];
+#pragma unroll
+  for (int u = 0; u < TPW2; u++) {
+    const int r = wave + 4 * u;
+    if (r >= RT2) continue;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 16 * r + kl + 4 * i;
+      if (row < ne) out[row] = s2 * acc2[u][i];
+    }
   }
 }
 
@@ -1356,18 +1683,18 @@ inline GemmShape gemm_shape(int ncols, int M, int K) {
   return s;
 }
 
-/* Launches k_gemm_cols (and k_gemm_reduce when K is split; ws must then
- * hold nsplit * ncols * ldw doubles). */
+/* Launches k_gemm_cols; when K is split, ws must hold nsplit * ncols * ldw
+ * doubles and tile_cnt col_blocks * row_groups zeroed ints (left zeroed). */
 inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, int K, const double* A, int lda,
                                    int ncols, const int* in_off, const int* out_off, const int* sexp,
-                                   const double* in, double* out, int accumulate, double* ws) {
+                                   const double* in, double* out, int accumulate, double* ws, int* tile_cnt) {
   const dim3 grid(sh.col_blocks, sh.row_groups, sh.nsplit);
   double* w = sh.nsplit > 1 ? ws : nullptr;
   const int ks = sh.nsplit > 1 ? sh.ksplit : K;
 #define KIFMM_GEMM_CASE(R, C)                                                                                  \
   if (sh.rt == R && sh.ct == C)                                                                                \
     hipLaunchKernelGGL((k_gemm_cols<R, C>), grid, dim3(256), 0, st, M, K, A, lda, ncols, in_off, out_off, sexp, \
-                       in, out, accumulate, ks, w, sh.ldw);
+                       in, out, accumulate, ks, w, sh.ldw, tile_cnt);
   KIFMM_GEMM_CASE(2, 2)
   else KIFMM_GEMM_CASE(4, 2)
   else KIFMM_GEMM_CASE(4, 1)
@@ -1376,11 +1703,6 @@ inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, i
   else KIFMM_GEMM_CASE(10, 1)
   else return hipErrorInvalidValue;
 #undef KIFMM_GEMM_CASE
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess || sh.nsplit == 1) return e;
-  const size_t n = (size_t)ncols * M;
-  hipLaunchKernelGGL(k_gemm_reduce, dim3((int)min((size_t)2048, (n + 255) / 256)), dim3(256), 0, st, M, ncols,
-                     sh.nsplit, ws, sh.ldw, out_off, sexp, out, accumulate);
   return hipGetLastError();
 }
 
@@ -2167,6 +2489,14 @@ static __global__ void k_join_pot(double* __restrict__ pot, const double* __rest
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   const size_t stride = (size_t)gridDim.x * blockDim.x;
   for (; i < n; i += stride) pot[i] = u[i] + pot[i];
+}
+
+/* (x, y, z, w) records from coordinate arrays; w = 0 without a density array */
+static __global__ void k_pack4(double4* __restrict__ out, const double* __restrict__ x, const double* __restrict__ y,
+                               const double* __restrict__ z, const double* __restrict__ w, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) out[i] = make_double4(x[i], y[i], z[i], w ? w[i] : 0.0);
 }
 
 /* new densities (leaf order) into the w field of the source records */

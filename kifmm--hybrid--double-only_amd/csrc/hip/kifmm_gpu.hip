@@ -28,6 +28,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <thread>
 #include <vector>
@@ -38,6 +39,14 @@
 
 using namespace kifmm_dev;
 using kifmm::set_error;
+
+/* Device scratch of kifmm_gpu_direct_noalloc: the staged coordinates of up
+ * to KIFMM_DIRECT_NOALLOC_MAX points and their sums.  Every device the
+ * library runs on holds its own copy until the process ends.  In host code
+ * the array's name is not a device address: the per-device address comes
+ * from hipGetSymbolAddress (scratch_address, called by kifmm_gpu_create with
+ * the handle's device current) and only that address is passed on. */
+static __device__ __align__(256) double scratch[4 * KIFMM_DIRECT_NOALLOC_MAX];
 
 namespace {
 
@@ -81,7 +90,7 @@ struct kifmm_gpu {
   float *surf_uc32 = nullptr, *surf_de32 = nullptr;
   float4* src32 = nullptr;
   int *cube2surf = nullptr, *surf2cube = nullptr;
-  int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *depth = nullptr;
+  int *child = nullptr, *snum = nullptr, *sbeg = nullptr, *tbeg = nullptr, *depth = nullptr, *tag = nullptr;
   int *widx = nullptr, *xidx = nullptr;
   int *u_ptr = nullptr, *u_idx = nullptr; /* U-list CSR (node ids), for the close-pair search */
   /* work lists */
@@ -118,6 +127,8 @@ struct kifmm_gpu {
   size_t chk_cols = 0;
   double* ws = nullptr;
   size_t ws_len = 0;
+  int* tile_cnt = nullptr; /* split-K GEMMs: per output tile, the slices done (0 between launches) */
+  size_t tile_cnt_len = 0;
   kifmm_work_counts counts{};
   /* wall ms of kifmm_gpu_create: host plan, upload, close-pair search,
    * other device set-up (translation spectra, Gram records), whole call */
@@ -127,7 +138,7 @@ struct kifmm_gpu {
   /* HIP graph of the whole launch sequence (KIFMM_GPU_GRAPH) */
   int use_graph = 0;
   bool capturing = false;
-  hipGraph_t graph[2] = {nullptr, nullptr}; /* [1]: with the copy-out */
+  hipGraph_t graph[2] = {nullptr, nullptr}; /* [0]: the far-field sequence */
   hipGraphExec_t gexec[2] = {nullptr, nullptr};
   /* copy-out: target cuts of the U-list chunks (leaf starts, 0 .. n_trg),
    * the first item of each chunk per item list, the copy stream (default
@@ -141,6 +152,8 @@ struct kifmm_gpu {
   double* potU = nullptr;
   /* staging of kifmm_gpu_set_densities */
   double* den_stage = nullptr;
+  /* this device's address of the file-scope scratch array */
+  double* scratch = nullptr;
 };
 
 namespace {
@@ -213,6 +226,30 @@ int find_box(const kifmm_tree_view* t, int d, const int want[3]) {
   return cur;
 }
 
+/* The per-device address of `scratch`, looked up once per device (the
+ * device must be current) and kept. */
+int scratch_address(int dev, double** out) {
+  static std::mutex mu;
+  static std::vector<double*> addr;
+  std::lock_guard<std::mutex> lock(mu);
+  if (dev < 0) {
+    set_error("scratch_address: invalid device");
+    return KIFMM_EINVAL;
+  }
+  if ((size_t)dev >= addr.size()) addr.resize(dev + 1, nullptr);
+  if (!addr[dev]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+      set_error(std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+      return KIFMM_EDEVICE;
+    }
+    addr[dev] = (double*)p;
+  }
+  *out = addr[dev];
+  return KIFMM_OK;
+}
+
 /* Structural checks of a caller-supplied tree view: every index a kernel
  * derives from it must land inside the arrays it addresses. */
 int check_view(const kifmm_tree_view* t) {
@@ -273,7 +310,7 @@ struct Plan {
   HostGemm s2m_gemm;
   std::vector<HostLevel> lv;
   size_t chk_cols = 0;
-  size_t ws_len = 0;
+  size_t ws_len = 0, tile_cnt_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
   std::vector<int4> witems, xitems, wred, xred; /* two int4 per item */
   int wx_slots = 0, w_slots = 0, x_slots = 0;     /* partial slots of split W / X lists */
@@ -437,12 +474,26 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     }
     P.chk_cols = std::max(P.chk_cols, (size_t)col);
   }
+  /* the fused downward kernel (k_l2l_dc2de) of depth d >= 3 runs over the
+   * L2L parents at d - 1 and writes the densities of their target
+   * children: exactly the DC2DE list of depth d */
+  for (int d = 3; d <= maxd; d++) {
+    size_t kids = 0;
+    for (int off : P.lv[d - 1].l2l.out) {
+      const int c0 = off / lde;
+      for (int o = 0; o < 8; o++) kids += (t->tag[c0 + o] & KIFMM_TRGNODE) != 0;
+    }
+    if (kids != P.lv[d].dc2de.in.size()) return fail("L2L parents do not cover the DC2DE nodes of a level");
+  }
   /* split-K workspace: the largest nsplit * ncols * ldw of any launch */
   {
     auto need = [&](size_t ncols, int M, int K) {
       if (!ncols) return;
       const GemmShape sh = gemm_shape((int)ncols, M, K);
-      if (sh.nsplit > 1) P.ws_len = std::max(P.ws_len, (size_t)sh.nsplit * ncols * sh.ldw);
+      if (sh.nsplit > 1) {
+        P.ws_len = std::max(P.ws_len, (size_t)sh.nsplit * ncols * sh.ldw);
+        P.tile_cnt_len = std::max(P.tile_cnt_len, (size_t)sh.col_blocks * sh.row_groups);
+      }
     };
     need(P.s2m_gemm.in.size(), ne, lduc);
     for (const HostLevel& L : P.lv) {
@@ -648,14 +699,41 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
   g->max_depth = P.max_depth;
   g->counts = P.wc;
 
-  /* points */
+  /* points: the caller's coordinate arrays go up as they are and are
+   * packed into (x, y, z, w) records on the device (k_pack4) */
   {
-    std::vector<double4> s(t->n_src), q(t->n_trg);
-    for (int i = 0; i < t->n_src; i++) s[i] = make_double4(t->sx[i], t->sy[i], t->sz[i], t->sden[i]);
-    for (int i = 0; i < t->n_trg; i++) q[i] = make_double4(t->tx[i], t->ty[i], t->tz[i], 0.0);
-    TRY(dev_upload(g, &g->src4, s.data(), s.size()));
-    TRY(dev_upload(g, &g->trg4, q.data(), q.size()));
+    TRY(dev_zero(g, &g->src4, (size_t)t->n_src));
+    TRY(dev_zero(g, &g->trg4, (size_t)t->n_trg));
     TRY(dev_zero(g, &g->pot, (size_t)t->n_trg));
+    const size_t m = (size_t)std::max(std::max(t->n_src, t->n_trg), 1);
+    double* stage = nullptr;
+    if (hipMalloc(&stage, 4 * m * sizeof(double)) != hipSuccess) {
+      set_error("hipMalloc of the point staging buffer failed");
+      return KIFMM_ENOMEM;
+    }
+    struct Free {
+      double* p;
+      ~Free() { (void)hipFree(p); }
+    } free_stage{stage};
+    const int blocks = (int)std::min<size_t>(4096, (m + 255) / 256);
+    const double* srcs[4] = {t->sx, t->sy, t->sz, t->sden};
+    const double* trgs[3] = {t->tx, t->ty, t->tz};
+    if (t->n_src > 0) {
+      for (int k = 0; k < 4; k++)
+        HIP_TRY(hipMemcpyAsync(stage + k * m, srcs[k], sizeof(double) * t->n_src, hipMemcpyHostToDevice, g->stream));
+      hipLaunchKernelGGL(k_pack4, dim3(blocks), dim3(256), 0, g->stream, g->src4, stage, stage + m, stage + 2 * m,
+                         stage + 3 * m, (size_t)t->n_src);
+      HIP_TRY(hipGetLastError());
+    }
+    if (t->n_trg > 0) {
+      HIP_TRY(hipStreamSynchronize(g->stream)); /* the stage is reused */
+      for (int k = 0; k < 3; k++)
+        HIP_TRY(hipMemcpyAsync(stage + k * m, trgs[k], sizeof(double) * t->n_trg, hipMemcpyHostToDevice, g->stream));
+      hipLaunchKernelGGL(k_pack4, dim3(blocks), dim3(256), 0, g->stream, g->trg4, stage, stage + m, stage + 2 * m,
+                         nullptr, (size_t)t->n_trg);
+      HIP_TRY(hipGetLastError());
+    }
+    HIP_TRY(hipStreamSynchronize(g->stream));
   }
   /* node geometry (fmm/input.cc:113-137) and node arrays */
   {
@@ -672,6 +750,7 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
     TRY(dev_upload(g, &g->sbeg, t->src_beg, nn));
     TRY(dev_upload(g, &g->tbeg, t->trg_beg, nn));
     TRY(dev_upload(g, &g->depth, t->depth, nn));
+    TRY(dev_upload(g, &g->tag, t->tag, nn));
     TRY(dev_upload(g, &g->widx, t->w_idx, t->w_ptr[nn]));
     TRY(dev_upload(g, &g->xidx, t->x_idx, t->x_ptr[nn]));
     TRY(dev_upload(g, &g->u_ptr, t->u_ptr, (size_t)nn + 1));
@@ -776,6 +855,8 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
   TRY(dev_zero(g, &g->chk, std::max<size_t>(P.chk_cols, 1) * lduc));
   g->ws_len = P.ws_len;
   TRY(dev_zero(g, &g->ws, std::max<size_t>(P.ws_len, 1)));
+  g->tile_cnt_len = P.tile_cnt_len;
+  TRY(dev_zero(g, &g->tile_cnt, std::max<size_t>(P.tile_cnt_len, 1)));
   g->n_vblocks = (int)P.vblocks.size();
   g->n_hpar = (int)P.hslot.size();
   TRY(dev_upload(g, &g->vblocks, P.vblocks.data(), P.vblocks.size()));
@@ -931,12 +1012,64 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
     set_error("internal: GEMM workspace too small");
     return KIFMM_EINVAL;
   }
+  if (sh.nsplit > 1 && (size_t)sh.col_blocks * sh.row_groups > g->tile_cnt_len) {
+    set_error("internal: GEMM tile counters too few");
+    return KIFMM_EINVAL;
+  }
   const hipError_t e = launch_gemm_cols(g->stream, sh, M, K, A, lda, L.n, L.in_off, L.out_off, L.sexp, in, out,
-                                        accumulate, g->ws);
+                                        accumulate, g->ws, g->tile_cnt);
   if (e != hipSuccess) {
     set_error(std::string("GEMM launch: ") + hipGetErrorString(e));
     return KIFMM_EDEVICE;
   }
+  return KIFMM_OK;
+}
+
+/* M2M of the parents at depth d (UE2UC stack, then UC2UE), fused.  A block
+ * runs every k step of both operators for its 16 columns, so the fused form
+ * is used where that stays short (NP <= 4) or where the level has enough
+ * columns to fill the chip (>= 2048); small NP = 6 levels keep the split-K
+ * column GEMMs. */
+bool fused_m2m(const kifmm_gpu* g, int ncols) { return g->np <= 4 || ncols >= 2048; }
+
+int m2m(kifmm_gpu* g, const GemmList& A, const GemmList& B, int d) {
+  const dim3 grid((A.n + 15) / 16);
+  const int rt1 = g->lda_uc / 16, rt2 = g->lda_e / 16;
+#define KIFMM_M2M_CASE(R1, R2)                                                                                    \
+  if (rt1 == R1 && rt2 == R2)                                                                                     \
+    hipLaunchKernelGGL((k_m2m<R1, R2>), grid, dim3(256), 0, g->stream, g->n_e, g->ld_e, g->ld_uc, g->UE2UC,        \
+                       g->lda_uc, g->UC2UE, g->lda_e, A.n, A.in_off, B.out_off, d, g->uden);
+  KIFMM_M2M_CASE(7, 2)
+  else KIFMM_M2M_CASE(10, 4)
+  else KIFMM_M2M_CASE(19, 10)
+  else {
+    set_error("internal: no fused upward kernel for these operator sizes");
+    return KIFMM_EINVAL;
+  }
+#undef KIFMM_M2M_CASE
+  LAUNCH_CHECK();
+  return KIFMM_OK;
+}
+
+/* L2L of the parents at depth d - 1 and DC2DE of their children, fused */
+int l2l_dc2de(kifmm_gpu* g, const GemmList& L, int d) {
+  if (L.n == 0) return KIFMM_OK;
+  const dim3 grid((L.n + 15) / 16, 8);
+  switch ((g->n_e + 15) / 16) {
+#define KIFMM_DOWN_CASE(R)                                                                                        \
+  case R:                                                                                                         \
+    hipLaunchKernelGGL(k_l2l_dc2de<R>, grid, dim3(256), 0, g->stream, g->n_e, g->ld_e, g->DE2DC, g->lda_l2l,       \
+                       g->DC2DE, g->lda_e, L.n, L.in_off, L.out_off, g->tag, d, g->dchk, g->dden);                 \
+    break;
+    KIFMM_DOWN_CASE(2)
+    KIFMM_DOWN_CASE(4)
+    KIFMM_DOWN_CASE(10)
+#undef KIFMM_DOWN_CASE
+    default:
+      set_error("internal: no fused downward kernel for this n_e");
+      return KIFMM_EINVAL;
+  }
+  LAUNCH_CHECK();
   return KIFMM_OK;
 }
 
@@ -996,25 +1129,28 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
  * <= 64 (list 1), the difference form and the mixed in-loop test items of
  * <= P2P_MAXT (list 0). */
 int u_list_index(const kifmm_gpu* g) {
-  if (g->precision == KIFMM_PREC_MIXED) return g->close_listed ? 2 : 0;
+  if (g->precision == KIFMM_PREC_MIXED) return g->close_listed && g->p2p_form != KIFMM_P2P_SPLIT ? 2 : 0;
   if (g->p2p_form == KIFMM_P2P_STATIC64) return 1;
   return g->trg_gram ? 2 : 0;
 }
 
-/* U-list over the items of copy-out chunk `chunk` (-1: all items) on stream
- * st, writing out[t] = u (accum = 0) or out[t] = u + out[t] (accum = 1). */
+/* U-list over the items of copy-out chunks [c0, c1) on stream st, writing
+ * out[t] = u (accum = 0) or out[t] = u + out[t] (accum = 1).  The
+ * persistent form's per-XCD counters of the launch are slot c0. */
 template <bool MASK>
-int launch_u(kifmm_gpu* g, hipStream_t st, int chunk, double* out, int accum) {
+int launch_u(kifmm_gpu* g, hipStream_t st, int c0, int c1, double* out, int accum) {
   const int l = u_list_index(g);
   const int4* list = l == 0 ? g->p2p_items : l == 1 ? g->p2p_items64 : g->p2p_items5;
-  const int n_list = l == 0 ? g->n_p2p : l == 1 ? g->n_p2p64 : g->n_p2p5;
-  const int beg = chunk < 0 ? 0 : g->chunk_beg[l][chunk];
-  const int end = chunk < 0 ? n_list : g->chunk_beg[l][chunk + 1];
+  const int beg = g->chunk_beg[l][c0], end = g->chunk_beg[l][c1];
+  const int chunk = c0;
   const int n = end - beg, per = (n + 7) / 8;
   if (n <= 0) return KIFMM_OK;
   const int4* items = list + beg;
   if (g->precision == KIFMM_PREC_MIXED) {
-    if (g->close_listed)
+    if (g->close_listed && g->p2p_form == KIFMM_P2P_SPLIT)
+      hipLaunchKernelGGL(k_p2p_ulist_mixed_split<MASK>, dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr,
+                         g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx, out, accum);
+    else if (g->close_listed)
       hipLaunchKernelGGL(k_p2p_ulist_mixed_tau<MASK>, dim3(8 * per), dim3(256), 0, st, items, n, per, g->rng_ptr,
                          g->rng, g->geo, g->src4, g->trg4, g->cptr, g->cidx, out, accum);
     else
@@ -1050,8 +1186,23 @@ int launch_u(kifmm_gpu* g, hipStream_t st, int chunk, double* out, int accum) {
   return KIFMM_OK;
 }
 
-int launch_u(kifmm_gpu* g, hipStream_t st, int chunk, double* out, int accum) {
-  return g->coincident ? launch_u<true>(g, st, chunk, out, accum) : launch_u<false>(g, st, chunk, out, accum);
+int launch_u(kifmm_gpu* g, hipStream_t st, int c0, int c1, double* out, int accum) {
+  return g->coincident ? launch_u<true>(g, st, c0, c1, out, accum) : launch_u<false>(g, st, c0, c1, out, accum);
+}
+
+/* Copy-out chunks whose U-list runs on the second stream from the start of
+ * the evaluation (alongside the far field) rather than after it: none in
+ * the default layout (tuning.overlap 0), all of them in the overlap layout
+ * (1), the first one or two (2, 3) in the split layouts. */
+int early_chunks(const kifmm_gpu* g, unsigned mask) {
+  if (!(mask & KIFMM_PHASE_U)) return 0;
+  const int nch = (int)g->cuts.size() - 1;
+  switch (g->overlap) {
+    case 1: return nch;
+    case 2: return std::min(1, nch);
+    case 3: return std::min(2, nch);
+    default: return 0;
+  }
 }
 
 template <bool MASK>
@@ -1089,15 +1240,32 @@ int copy_out(kifmm_gpu* g, hipStream_t st, double* dst, int b, int e) {
   return KIFMM_OK;
 }
 
-/* The launch sequence of one evaluation.  Order: up, V, W, X, down, then
- * the U-list, so that the U-list's accumulating write leaves each target
- * final and its chunk can go to the host (dst, when non-null) on the copy
- * stream while the later chunks compute; the chunks alternate between the
- * handle's two compute streams, so one chunk's tail overlaps the next
- * one's start.  With the overlap layout the U-list instead runs from the
- * start on the second stream into potU, alongside everything else, and
- * k_join_pot adds it in at the end (the same sum, operand for operand). */
-int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
+/* One evaluation is three launch sequences on the handle's streams:
+ *   enqueue_side  the U-list of the early copy-out chunks (early_chunks) on
+ *                 the second stream from the start, into potU, alongside
+ *                 the far field;
+ *   enqueue_far   the fills, up, V, W, X and down on the main stream (what a
+ *                 graph handle captures and replays);
+ *   enqueue_near  the join pot = potU + pot of the early chunks, then the
+ *                 U-list of the other chunks, alternating between the two
+ *                 compute streams, each chunk's potentials copied to the
+ *                 host (dst, when non-null) on the copy stream while the
+ *                 later chunks compute.
+ * The U-list's accumulating write (u + pot) and the join (potU + pot) add
+ * the same two terms, so every layout gives the same bits. */
+int enqueue_side(kifmm_gpu* g, unsigned mask) {
+  const int e = early_chunks(g, mask);
+  if (e == 0) return KIFMM_OK;
+  HIP_TRY(hipEventRecord(g->fork_ev, g->stream)); /* after the previous evaluation's join read potU */
+  HIP_TRY(hipStreamWaitEvent(g->side, g->fork_ev, 0));
+  HIP_TRY(hipEventRecord(g->su0, g->side));
+  TRY(launch_u(g, g->side, 0, e, g->potU, 0));
+  HIP_TRY(hipEventRecord(g->su1, g->side));
+  HIP_TRY(hipEventRecord(g->join_ev, g->side));
+  return KIFMM_OK;
+}
+
+int enqueue_far(kifmm_gpu* g, unsigned mask) {
   hipEvent_t* ev = g->ev;
   const int lde = g->ld_e, lduc = g->ld_uc;
   /* zero what this call accumulates into (fill kernels, so the sequence
@@ -1108,16 +1276,6 @@ int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
   if (!(mask & KIFMM_PHASE_UP)) TRY(fill(g, g->uden, (size_t)g->n_nodes * lde));
   TRY(fill(g, g->dchk, (size_t)g->n_nodes * lde));
   if (!(mask & KIFMM_PHASE_DOWN)) TRY(fill(g, g->dden, (size_t)g->n_nodes * lde));
-  const bool use_u = (mask & KIFMM_PHASE_U) != 0;
-  const bool side_u = g->overlap && use_u;
-  if (side_u) {
-    HIP_TRY(hipEventRecord(g->fork_ev, g->stream));
-    HIP_TRY(hipStreamWaitEvent(g->side, g->fork_ev, 0));
-    if (!g->capturing) HIP_TRY(hipEventRecord(g->su0, g->side));
-    TRY(launch_u(g, g->side, -1, g->potU, 0));
-    if (!g->capturing) HIP_TRY(hipEventRecord(g->su1, g->side));
-    HIP_TRY(hipEventRecord(g->join_ev, g->side));
-  }
   /* up: S2M check potentials, UC2UE, then M2M bottom-up */
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
@@ -1136,8 +1294,13 @@ int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
     TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
   }
   for (int d = g->max_depth - 1; d >= 0 && (mask & KIFMM_PHASE_UP); d--) {
-    TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
-    TRY(gemm(g, g->lv[d].m2m_b, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
+    if (g->lv[d].m2m_a.n == 0) continue;
+    if (fused_m2m(g, g->lv[d].m2m_a.n)) {
+      TRY(m2m(g, g->lv[d].m2m_a, g->lv[d].m2m_b, d));
+    } else {
+      TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
+      TRY(gemm(g, g->lv[d].m2m_b, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
+    }
   }
   REC(ev[E_UP]);
   /* V-list */
@@ -1159,11 +1322,13 @@ int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
   else
     REC(ev[E_W]);
   REC(ev[E_X]);
-  /* down: per level L2L (depth >= 3) then DC2DE (depth >= 2), then L2T */
+  /* down: DC2DE at depth 2, then per level the fused L2L + DC2DE
+   * (k_l2l_dc2de over the parents at depth d - 1), then L2T */
   for (int d = 2; d <= g->max_depth && (mask & KIFMM_PHASE_DOWN); d++) {
     if (d >= 3)
-      TRY(gemm(g, g->lv[d - 1].l2l, 8 * lde, lde, g->DE2DC, g->lda_l2l, g->dden, g->dchk, 1));
-    TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
+      TRY(l2l_dc2de(g, g->lv[d - 1].l2l, d));
+    else
+      TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0) {
     if (g->precision == KIFMM_PREC_MIXED)
@@ -1176,42 +1341,56 @@ int enqueue(kifmm_gpu* g, unsigned mask, double* dst) {
   }
   LAUNCH_CHECK();
   REC(ev[E_DOWN]);
+  return KIFMM_OK;
+}
+
+int enqueue_near(kifmm_gpu* g, unsigned mask, double* dst) {
   const int n = g->n_trg;
-  if (side_u) {
-    HIP_TRY(hipStreamWaitEvent(g->stream, g->join_ev, 0));
-    const int blocks = (int)std::min<size_t>(4096, ((size_t)n + 255) / 256);
-    if (n > 0) hipLaunchKernelGGL(k_join_pot, dim3(std::max(blocks, 1)), dim3(256), 0, g->stream, g->pot, g->potU, (size_t)n);
-    LAUNCH_CHECK();
-    REC(ev[E_U]);
+  if (!(mask & KIFMM_PHASE_U)) {
+    HIP_TRY(hipEventRecord(g->ev[E_U], g->stream));
     if (dst) TRY(copy_out(g, g->stream, dst, 0, n));
-  } else if (use_u) {
-    const int nch = dst ? (int)g->cuts.size() - 1 : 1;
-    if (nch > 1) {
+    return KIFMM_OK;
+  }
+  const int nch = (int)g->cuts.size() - 1, e = early_chunks(g, mask);
+  if (e > 0) { /* the early chunks: join, then their copy */
+    const size_t m = (size_t)g->cuts[e];
+    HIP_TRY(hipStreamWaitEvent(g->stream, g->join_ev, 0));
+    const int blocks = (int)std::min<size_t>(4096, (m + 255) / 256);
+    if (m > 0) hipLaunchKernelGGL(k_join_pot, dim3(std::max(blocks, 1)), dim3(256), 0, g->stream, g->pot, g->potU, m);
+    LAUNCH_CHECK();
+    if (dst) {
+      HIP_TRY(hipEventRecord(g->ch_ev[0], g->stream));
+      HIP_TRY(hipStreamWaitEvent(g->cstream, g->ch_ev[0], 0));
+      TRY(copy_out(g, g->cstream, dst, 0, (int)m));
+      for (int c = 0; c < e; c++) HIP_TRY(hipEventRecord(g->cd_ev[c], g->cstream));
+    }
+  }
+  if (e < nch) { /* the other chunks after the far field */
+    const int late = dst ? nch - e : 1; /* without a copy: one launch */
+    if (late > 1) {
       HIP_TRY(hipEventRecord(g->far_ev, g->stream));
       HIP_TRY(hipStreamWaitEvent(g->side, g->far_ev, 0));
     }
-    for (int c = 0; c < nch; c++) {
-      hipStream_t st = (c & 1) ? g->side : g->stream;
-      TRY(launch_u(g, st, nch > 1 ? c : -1, g->pot, 1));
+    for (int k = 0; k < late; k++) {
+      const int c = e + k;
+      hipStream_t st = (k & 1) ? g->side : g->stream;
+      TRY(launch_u(g, st, c, dst ? c + 1 : nch, g->pot, 1));
       if (dst) {
         HIP_TRY(hipEventRecord(g->ch_ev[c], st));
         HIP_TRY(hipStreamWaitEvent(g->cstream, g->ch_ev[c], 0));
-        TRY(copy_out(g, g->cstream, dst, nch > 1 ? g->cuts[c] : 0, nch > 1 ? g->cuts[c + 1] : n));
-        if (!g->capturing) HIP_TRY(hipEventRecord(g->cd_ev[c], g->cstream));
+        TRY(copy_out(g, g->cstream, dst, g->cuts[c], g->cuts[c + 1]));
+        HIP_TRY(hipEventRecord(g->cd_ev[c], g->cstream));
       }
     }
-    if (nch > 1) {
+    if (late > 1) {
       HIP_TRY(hipEventRecord(g->sdone_ev, g->side));
       HIP_TRY(hipStreamWaitEvent(g->stream, g->sdone_ev, 0));
     }
-    REC(ev[E_U]);
-    if (dst) {
-      HIP_TRY(hipEventRecord(g->cdone_ev, g->cstream));
-      HIP_TRY(hipStreamWaitEvent(g->stream, g->cdone_ev, 0));
-    }
-  } else {
-    REC(ev[E_U]);
-    if (dst) TRY(copy_out(g, g->stream, dst, 0, n));
+  }
+  HIP_TRY(hipEventRecord(g->ev[E_U], g->stream));
+  if (dst) {
+    HIP_TRY(hipEventRecord(g->cdone_ev, g->cstream));
+    HIP_TRY(hipStreamWaitEvent(g->stream, g->cdone_ev, 0));
   }
   return KIFMM_OK;
 }
@@ -1235,16 +1414,19 @@ void host_copy(double* to, const double* from, size_t b, size_t e) {
 int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* times) {
   hipEvent_t* ev = g->ev;
   HIP_TRY(hipSetDevice(g->dev));
+  /* a graph handle replays the far-field sequence (fills, up, V, W, X,
+   * down) as one graph: HIP runs a graph's nodes in one order on the launch
+   * stream, so the U-list chunks and their copies, whose point is to run
+   * beside one another, stay outside it */
   const bool graph = g->use_graph && mask == KIFMM_PHASE_ALL;
   /* the copy lands in the handle's pinned buffer (the caller's own buffer
-   * when it is that one; otherwise one host pass follows); without pinned
-   * memory, eagerly into the caller's buffer, or after a graph */
-  double* dst = pot_leaf ? (g->pinned ? g->pinned : (graph ? nullptr : pot_leaf)) : nullptr;
-  const int gi = dst ? 1 : 0;
-  if (graph && !g->gexec[gi]) {
+   * when it is that one; otherwise one host pass follows), or without
+   * pinned memory straight into the caller's buffer */
+  double* dst = pot_leaf ? (g->pinned ? g->pinned : pot_leaf) : nullptr;
+  if (graph && !g->gexec[0]) {
     g->capturing = true;
     HIP_TRY(hipStreamBeginCapture(g->stream, hipStreamCaptureModeThreadLocal));
-    const int r = enqueue(g, mask, dst);
+    const int r = enqueue_far(g, mask);
     hipGraph_t gr = nullptr;
     const hipError_t e = hipStreamEndCapture(g->stream, &gr);
     g->capturing = false;
@@ -1253,23 +1435,23 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
       return r;
     }
     HIP_TRY(e);
-    g->graph[gi] = gr;
-    HIP_TRY(hipGraphInstantiate(&g->gexec[gi], gr, nullptr, nullptr, 0));
+    g->graph[0] = gr;
+    HIP_TRY(hipGraphInstantiate(&g->gexec[0], gr, nullptr, nullptr, 0));
   }
   HIP_TRY(hipEventRecord(ev[E_START], g->stream));
+  TRY(enqueue_side(g, mask));
   if (graph) {
-    HIP_TRY(hipGraphLaunch(g->gexec[gi], g->stream));
-    for (int i = E_UP; i <= E_U; i++) HIP_TRY(hipEventRecord(ev[i], g->stream));
+    HIP_TRY(hipGraphLaunch(g->gexec[0], g->stream));
+    for (int i = E_UP; i <= E_DOWN; i++) HIP_TRY(hipEventRecord(ev[i], g->stream));
   } else {
-    TRY(enqueue(g, mask, dst));
+    TRY(enqueue_far(g, mask));
   }
-  if (pot_leaf && !dst) /* a graph handle without pinned memory */
-    HIP_TRY(hipMemcpyAsync(pot_leaf, g->pot, sizeof(double) * g->n_trg, hipMemcpyDeviceToHost, g->stream));
+  TRY(enqueue_near(g, mask, dst));
   HIP_TRY(hipEventRecord(ev[E_END], g->stream));
   double host_copy_ms = 0.0;
   if (pot_leaf && dst == g->pinned && pot_leaf != g->pinned) {
     const int nch = (int)g->cuts.size() - 1;
-    const bool pipelined = !graph && !g->overlap && (mask & KIFMM_PHASE_U) && nch > 1;
+    const bool pipelined = (mask & KIFMM_PHASE_U) && nch > 1; /* cd_ev recorded for every chunk */
     double t0 = 0.0;
     if (pipelined) { /* each chunk as soon as its copy is done */
       for (int c = 0; c < nch; c++) {
@@ -1301,15 +1483,18 @@ int evaluate(kifmm_gpu* g, unsigned mask, double* pot_leaf, kifmm_phase_times* t
       times->wlist = ms[E_W];
       times->xlist = ms[E_X];
       times->down = ms[E_DOWN];
-      times->ulist = times->p2p_kernel = ms[E_U];
-      if (g->overlap && (mask & KIFMM_PHASE_U)) { /* the U-list ran on the side stream */
-        float mu = 0;
-        HIP_TRY(hipEventElapsedTime(&mu, g->su0, g->su1));
-        times->ulist = times->p2p_kernel = mu;
-      }
+    }
+    /* the U-list: its chunks after the far field, plus the span of the
+     * early chunks on the second stream (which overlaps the far field) */
+    times->ulist = times->p2p_kernel = ms[E_U];
+    if (early_chunks(g, mask) > 0) {
+      float mu = 0;
+      HIP_TRY(hipEventElapsedTime(&mu, g->su0, g->su1));
+      times->ulist += mu;
+      times->p2p_kernel = early_chunks(g, mask) == (int)g->cuts.size() - 1 ? mu : times->ulist;
     }
     /* the copy left after the last kernel (the rest overlaps the U-list) */
-    times->copy = (graph ? 0.0 : ms[E_END]) + host_copy_ms;
+    times->copy = ms[E_END] + host_copy_ms;
     times->total = tot + host_copy_ms;
   }
   return KIFMM_OK;
@@ -1398,6 +1583,10 @@ int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops, co
       hipEventCreateWithFlags(&g->cdone_ev, hipEventDisableTiming) != hipSuccess ||
       hipEventCreate(&g->su0) != hipSuccess || hipEventCreate(&g->su1) != hipSuccess) {
     set_error("kifmm_gpu_create: second stream, copy stream or their events could not be created");
+    kifmm_gpu_destroy(g);
+    return KIFMM_EDEVICE;
+  }
+  if (scratch_address(g->dev, &g->scratch) != KIFMM_OK) {
     kifmm_gpu_destroy(g);
     return KIFMM_EDEVICE;
   }
@@ -1519,6 +1708,37 @@ int kifmm_gpu_direct(kifmm_gpu* g, int nt, const double* tx, const double* ty, c
   return done(KIFMM_OK);
 }
 
+int kifmm_gpu_direct_noalloc(kifmm_gpu* g, int nt, const double* tx, const double* ty, const double* tz,
+                             double* out) {
+  if (!g || nt <= 0 || !tx || !ty || !tz || !out) {
+    set_error("kifmm_gpu_direct_noalloc: invalid argument");
+    return KIFMM_EINVAL;
+  }
+  if (nt > KIFMM_DIRECT_NOALLOC_MAX) {
+    set_error("kifmm_gpu_direct_noalloc: " + std::to_string(nt) + " points exceed the scratch of " +
+              std::to_string(KIFMM_DIRECT_NOALLOC_MAX) + " (use kifmm_gpu_direct)");
+    return KIFMM_EINVAL;
+  }
+  if (!g->scratch) {
+    set_error("kifmm_gpu_direct_noalloc: no scratch address for this device");
+    return KIFMM_EDEVICE;
+  }
+  HIP_TRY(hipSetDevice(g->dev));
+  /* targets at d, d + nt, d + 2 nt, sums at d + 3 nt: inside 4 * MAX doubles;
+   * the kernel assigns every sum, so nothing needs clearing */
+  double* d = g->scratch;
+  const size_t b = sizeof(double) * nt;
+  HIP_TRY(hipMemcpyAsync(d, tx, b, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(d + nt, ty, b, hipMemcpyHostToDevice, g->stream));
+  HIP_TRY(hipMemcpyAsync(d + 2 * (size_t)nt, tz, b, hipMemcpyHostToDevice, g->stream));
+  hipLaunchKernelGGL(k_direct, dim3(std::min(nt, 65535)), dim3(256), 0, g->stream, nt, d, d + nt, d + 2 * (size_t)nt,
+                     g->src4, g->n_src, d + 3 * (size_t)nt);
+  HIP_TRY(hipGetLastError());
+  HIP_TRY(hipMemcpyAsync(out, d + 3 * (size_t)nt, b, hipMemcpyDeviceToHost, g->stream));
+  HIP_TRY(hipStreamSynchronize(g->stream));
+  return KIFMM_OK;
+}
+
 double kifmm_problem_validate_gpu(kifmm_problem* p, kifmm_gpu* g, const double* pot_orig, int nv, int* chk_ids,
                                   double* direct_out) {
   if (!p || !g || !pot_orig || nv <= 0) {
@@ -1567,16 +1787,22 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
     set_error("kifmm_gpu_set_tuning: null argument");
     return KIFMM_EINVAL;
   }
-  if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_STATIC64 ||
+  if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_SPLIT ||
       (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
       (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA) ||
-      (t->overlap != 0 && t->overlap != 1)) {
+      t->overlap < 0 || t->overlap > 3) {
     set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16, "
-              "hadamard_form a KIFMM_HAD_* value and overlap 0 or 1");
+              "hadamard_form a KIFMM_HAD_* value and overlap 0 to 3");
     return KIFMM_EINVAL;
   }
-  if (t->p2p_form != KIFMM_P2P_STATIC && g->precision != KIFMM_PREC_FP64) {
-    set_error("kifmm_gpu_set_tuning: the U-list forms apply to the FP64 path");
+  if (g->precision == KIFMM_PREC_MIXED && t->p2p_form != KIFMM_P2P_STATIC &&
+      !(t->p2p_form == KIFMM_P2P_SPLIT && g->close_listed)) {
+    set_error("kifmm_gpu_set_tuning: the mixed path has the STATIC and, with a close-pair list, the SPLIT "
+              "U-list forms");
+    return KIFMM_EUNSUP;
+  }
+  if (g->precision == KIFMM_PREC_FP64 && t->p2p_form == KIFMM_P2P_SPLIT) {
+    set_error("kifmm_gpu_set_tuning: the SPLIT U-list form is the mixed path's");
     return KIFMM_EUNSUP;
   }
   if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb && t->hadamard_form == g->had_form &&
@@ -1613,7 +1839,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[11] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  double t[14] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
   auto time_phase = [&](unsigned phase, double* out) -> int {
@@ -1629,13 +1855,23 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     return KIFMM_OK;
   };
   kifmm_gpu_tuning c = cur;
-  const int nforms = g->precision == KIFMM_PREC_FP64 ? 3 : 1;
-  int best_form = 0;
-  for (int f = 0; f < nforms; f++) {
-    c.p2p_form = f;
+  /* the U-list forms of the handle's path, with their slot in t */
+  std::vector<std::pair<int, int>> forms;
+  if (g->precision == KIFMM_PREC_FP64)
+    forms = {{KIFMM_P2P_STATIC, 0}, {KIFMM_P2P_PERSISTENT, 1}, {KIFMM_P2P_STATIC64, 2}};
+  else if (g->close_listed)
+    forms = {{KIFMM_P2P_STATIC, 0}, {KIFMM_P2P_SPLIT, 11}};
+  else
+    forms = {{KIFMM_P2P_STATIC, 0}};
+  int best_form = KIFMM_P2P_STATIC, best_slot = 0;
+  for (const auto& f : forms) {
+    c.p2p_form = f.first;
     TRY(kifmm_gpu_set_tuning(g, &c));
-    TRY(time_phase(KIFMM_PHASE_U, &t[f]));
-    if (t[f] < t[best_form]) best_form = f;
+    TRY(time_phase(KIFMM_PHASE_U, &t[f.second]));
+    if (t[f.second] < t[best_slot]) {
+      best_form = f.first;
+      best_slot = f.second;
+    }
   }
   c.p2p_form = best_form;
   const int fbs[3] = {4, 8, 16};
@@ -1653,16 +1889,18 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
    * would otherwise capture here) */
   const int use_graph = g->use_graph;
   g->use_graph = 0;
+  /* slots of the layouts 0 .. 3 in t */
+  const int lslot[4] = {9, 10, 12, 13};
   int best_l = 0, rc = KIFMM_OK;
-  for (int l = 0; l < 2 && rc == KIFMM_OK; l++) {
+  for (int l = 0; l < 4 && rc == KIFMM_OK; l++) {
     c.overlap = l;
     rc = kifmm_gpu_set_tuning(g, &c);
     kifmm_phase_times pt{};
     for (int r = 0; r <= reps && rc == KIFMM_OK; r++) {
       rc = evaluate(g, KIFMM_PHASE_ALL, g->pinned, &pt); /* with the copy-out, as the layouts differ there */
-      if (r == 1 || (r > 1 && pt.total < t[9 + l])) t[9 + l] = pt.total; /* run 0 warms up */
+      if (r == 1 || (r > 1 && pt.total < t[lslot[l]])) t[lslot[l]] = pt.total; /* run 0 warms up */
     }
-    if (rc == KIFMM_OK && t[9 + l] < t[9 + best_l]) best_l = l;
+    if (rc == KIFMM_OK && t[lslot[l]] < t[lslot[best_l]]) best_l = l;
   }
   g->use_graph = use_graph;
   TRY(rc);
@@ -1670,7 +1908,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 11); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 14); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

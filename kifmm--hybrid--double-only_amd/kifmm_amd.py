@@ -76,7 +76,7 @@ class GpuTuning(C.Structure):
                 ("overlap", C.c_int), ("reserved", C.c_int * 4)]
 
 
-P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64 = 0, 1, 2
+P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64, P2P_SPLIT = 0, 1, 2, 3
 HAD_VALU, HAD_MFMA = 0, 1
 
 
@@ -140,6 +140,7 @@ _sig("kifmm_gpu_set_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
 _sig("kifmm_gpu_get_tuning", C.c_int, [C.c_void_p, C.POINTER(GpuTuning)])
 _sig("kifmm_gpu_autotune", C.c_int, [C.c_void_p, C.c_int, C.POINTER(GpuTuning), _dp, C.c_int])
 _sig("kifmm_gpu_output", _dp, [C.c_void_p])
+_sig("kifmm_gpu_direct_noalloc", C.c_int, [C.c_void_p, C.c_int, _dp, _dp, _dp, _dp])
 _sig("kifmm_gpu_set_densities", C.c_int, [C.c_void_p, _dp])
 _sig("kifmm_gpu_create_times", C.c_int, [C.c_void_p, _dp, C.c_int])
 _sig("kifmm_gpu_close_list", C.c_long, [C.c_void_p, _ip, C.c_long, _ip, C.c_long])
@@ -402,6 +403,15 @@ class GpuEvaluator:
         _check(lib.kifmm_gpu_evaluate(self._h, _dptr(pot), C.byref(t)))
         return t.as_dict()
 
+    def direct(self, tx, ty, tz, noalloc=False):
+        """FP64 direct sums over this handle's sources at the given points
+        (kifmm_gpu_direct, or kifmm_gpu_direct_noalloc with noalloc=True)."""
+        t = [np.ascontiguousarray(a, dtype=np.float64) for a in (tx, ty, tz)]
+        out = np.zeros(t[0].size)
+        f = lib.kifmm_gpu_direct_noalloc if noalloc else lib.kifmm_gpu_direct
+        _check(f(self._h, t[0].size, *[_dptr(a) for a in t], _dptr(out)))
+        return out
+
     def set_densities(self, den_leaf):
         """New source densities in gathered source order (kifmm_gpu_set_densities)."""
         den = np.ascontiguousarray(den_leaf, dtype=np.float64)
@@ -451,10 +461,12 @@ class GpuEvaluator:
         """Time the launch shapes on this input, keep the fastest; returns
         (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, then the
         Hadamard on the VALU with 4, 8, 16 frequencies per block and on MFMA
-        with 4, 8, 16, then whole evaluations on one stream and overlapped."""
+        with 4, 8, 16, then whole evaluations with the copy back in stream
+        layouts 0 and 1, then SPLIT (mixed path), then layouts 2 and 3; -1
+        where not timed."""
         t = GpuTuning()
-        ms = np.zeros(11)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 11))
+        ms = np.zeros(14)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 14))
         return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
                 "overlap": t.overlap}, ms
 

@@ -35,21 +35,35 @@ def test_copy_out_is_the_device_result(prec):
     assert np.array_equal(own, dev)
     got, _ = ev.evaluate()  # numpy copy of the pinned buffer
     assert np.array_equal(got, dev)
-    ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, 1)  # the U-list on the second stream
-    got, _ = ev.evaluate()
-    assert np.array_equal(got, dev)
-    forms = (K.P2P_PERSISTENT, K.P2P_STATIC64) if prec == K.PREC_FP64 else ()
+    for ov in (1, 2, 3):  # U-list chunks on the second stream from the start: all, first, first two
+        ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, ov)
+        got, t = ev.evaluate()
+        assert np.array_equal(got, dev), ov
+        own[:] = np.nan
+        ev.evaluate_into(own)
+        assert np.array_equal(own, dev), ov
+        ev.evaluate(copy=False)
+        assert np.array_equal(ev.intermediate("pot"), dev), ov
+    forms = (K.P2P_PERSISTENT, K.P2P_STATIC64) if prec == K.PREC_FP64 else (K.P2P_SPLIT,)
+    split = None
     for f in forms:
-        for ov in (0, 1):
+        for ov in (0, 1, 2):
             ev.set_tuning(f, 8, K.HAD_MFMA, ov)
             got, _ = ev.evaluate()
-            assert np.array_equal(got, dev), (f, ov)
+            if f == K.P2P_SPLIT:  # its own sums (rounding-level differences), the same in every layout
+                split = got if split is None else split
+                assert np.array_equal(got, split), (f, ov)
+            else:
+                assert np.array_equal(got, dev), (f, ov)
     gr = K.GpuEvaluator(prob, precision=prec, graph=True)
     for k in range(3):
+        gr.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, k)  # a recapture per layout
         got, _ = gr.evaluate()
         assert np.array_equal(got, dev), ("graph replay", k)
         gr.evaluate(copy=False)
         assert np.array_equal(gr.intermediate("pot"), dev)
+        got, _ = gr.evaluate()
+        assert np.array_equal(got, dev), ("graph replay", k)
         ev.evaluate()
 
 

@@ -84,7 +84,7 @@ def test_graph_replays_match_eager():
         for _ in range(3):
             got, t = graph.evaluate()
             assert np.array_equal(got, ref)
-            assert t["total"] > 0 and t["ulist"] == 0
+            assert t["total"] > 0 and t["up"] == 0 and t["ulist"] > 0  # far field inside the graph
             again, _ = eager.evaluate()
             assert np.array_equal(again, ref)
 
@@ -137,3 +137,45 @@ def test_dense_close_pairs_fall_back_to_the_inline_test():
     e, _ = K.GpuEvaluator(prob, inline_close=True).evaluate()
     assert np.array_equal(a, b) and np.array_equal(d, e)
     assert rel(a, d) < 1e-5
+
+
+@pytest.mark.parametrize("case", [(65536, "uniform", 64, 4, 0, 1), (120000, "uniform", 128, 4, 1, 0),
+                                  (65536, "ellipsoid", 32, 4, 2, 0), (20000, "gaussmix", 32, 3, 3, 1)],
+                         ids=lambda c: f"{c[1]}-n{c[0]}-q{c[2]}-np{c[3]}-C{c[5]}")
+def test_split_u_list_form_matches_oracle(case):
+    """KIFMM_P2P_SPLIT (groups of a set share each source tile among their
+    waves): against the FP64 oracle at the mixed bar, against the default
+    mixed form at rounding level (other partial sums), repeatable bit for
+    bit, eagerly and in a graph."""
+    n, dist, q, np_, seed, c = case
+    print(f"case split n={n} dist={dist} q={q} NP={np_} C={c}", flush=True)
+    prob = K.Problem.create(n, dist, q, np_=np_, seed=seed, coincide=c)
+    ev = K.GpuEvaluator(prob, precision=K.PREC_MIXED)
+    tau, _ = ev.evaluate()
+    ev.set_tuning(K.P2P_SPLIT, 8, K.HAD_MFMA)
+    split, _ = ev.evaluate()
+    again, _ = ev.evaluate()
+    assert np.array_equal(split, again)
+    gr = K.GpuEvaluator(prob, precision=K.PREC_MIXED, graph=True)
+    gr.set_tuning(K.P2P_SPLIT, 8, K.HAD_MFMA)
+    for _ in range(2):
+        assert np.array_equal(gr.evaluate()[0], split)
+    orc = O.Oracle(n, dist, q, np_, seed, c)
+    o = prob.operators()
+    orc.set_operators(*[np.ctypeslib.as_array(p, shape=(k,)).copy() for p, k in
+                        ((o.uc2ue, o.n_e * o.n_uc), (o.ue2uc, 8 * o.n_uc * o.n_e), (o.dc2de, o.n_e ** 2),
+                         (o.de2dc, 8 * o.n_e ** 2))])
+    ref, _ = orc.evaluate(O.KERNEL_EXACT)
+    r, d = rel(prob.scatter(split), ref), rel(split, tau)
+    print(f"  split vs oracle {r:.3e}, split vs tau form {d:.3e}", flush=True)
+    assert np.all(np.isfinite(split)) and r < 1e-5 and d < 1e-6
+
+
+def test_split_form_needs_the_mixed_path_and_a_close_list():
+    prob = K.Problem.create(20000, "uniform", 64, np_=4, seed=8)
+    with pytest.raises(K.KifmmError) as e:
+        K.GpuEvaluator(prob).set_tuning(K.P2P_SPLIT, 8)
+    assert e.value.code == -4
+    with pytest.raises(K.KifmmError) as e:
+        K.GpuEvaluator(prob, precision=K.PREC_MIXED, inline_close=True).set_tuning(K.P2P_SPLIT, 8)
+    assert e.value.code == -4

@@ -42,7 +42,7 @@ def test_every_shape_gives_identical_potentials(n, dist, q, np_, c):
     ref_mfma = None
     for form in (K.HAD_VALU, K.HAD_MFMA):
         for k, (f, fb) in enumerate(SHAPES):
-            ov = (k + form) % 2  # both stream layouts across the shapes
+            ov = (k + form) % 4  # every stream layout across the shapes
             ev.set_tuning(f, fb, form, ov)
             assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb, "hadamard_form": form, "overlap": ov}
             got, _ = ev.evaluate()
@@ -84,8 +84,8 @@ def test_mixed_path_hadamard_blocks_and_forms():
 
 def test_invalid_tuning_is_rejected():
     ev = K.GpuEvaluator(K.Problem.create(5000, "uniform", 64, np_=4, seed=6))
-    for f, fb, form, ov in ((3, 8, 0, 0), (-1, 8, 0, 0), (0, 5, 0, 0), (0, 0, 0, 0), (0, 8, 2, 0), (0, 8, -1, 0),
-                            (0, 8, 1, 2), (0, 8, 1, -1)):
+    for f, fb, form, ov in ((4, 8, 0, 0), (-1, 8, 0, 0), (0, 5, 0, 0), (0, 0, 0, 0), (0, 8, 2, 0), (0, 8, -1, 0),
+                            (0, 8, 1, 4), (0, 8, 1, -1)):
         with pytest.raises(K.KifmmError) as e:
             ev.set_tuning(f, fb, form, ov)
         assert e.value.code == -1
@@ -99,12 +99,18 @@ def test_autotune_keeps_the_fastest(precision):
     ref, _ = ev.evaluate()
     best, ms = ev.autotune(reps=2)
     print(f"  times {np.round(ms, 4)} -> {best}", flush=True)
-    nforms = 3 if precision == K.PREC_FP64 else 1
-    assert np.all(ms[:nforms] > 0) and np.all(ms[nforms:3] == -1) and np.all(ms[3:] > 0)
-    assert best["p2p_form"] == int(np.argmin(ms[:nforms]))
+    if precision == K.PREC_FP64:
+        forms = {K.P2P_STATIC: 0, K.P2P_PERSISTENT: 1, K.P2P_STATIC64: 2}
+        assert ms[11] == -1
+    else:
+        forms = {K.P2P_STATIC: 0, K.P2P_SPLIT: 11}
+        assert np.all(ms[1:3] == -1)
+    assert all(ms[k] > 0 for k in forms.values()) and np.all(ms[3:11] > 0) and np.all(ms[12:14] > 0)
+    assert best["p2p_form"] == min(forms, key=lambda f: ms[forms[f]])
     v = int(np.argmin(ms[3:9]))
     assert (best["hadamard_form"], best["hadamard_fb"]) == (v // 3, (4, 8, 16)[v % 3])
-    assert best["overlap"] == int(np.argmin(ms[9:11]))
+    layouts = [ms[9], ms[10], ms[12], ms[13]]
+    assert best["overlap"] == int(np.argmin(layouts))
     assert ev.tuning() == best
     got, _ = ev.evaluate()
     assert rel(got, ref) < (O.PARITY_RTOL[4] if precision == K.PREC_FP64 else 1e-6)
