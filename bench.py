@@ -55,8 +55,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--nv", type=int, default=1000, help="validation samples, direct sums on the GPU (0 = skip)")
-    ap.add_argument("--only", choices=("headline", "np6"), default=None,
-                    help="run only the headline comparison or only the NP = 6 FP64 case (profiling)")
+    ap.add_argument("--only", choices=("headline", "np6", "nonuniform", "large"), default=None,
+                    help="run one case only (profiling): the headline comparison, the NP = 6 FP64 case, "
+                         "the autotuned ellipsoid or the 16M-point mixed-vs-FP64 case")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-tuned", action="store_true", help="skip the autotuned non-uniform case")
     ap.add_argument("--no-large", action="store_true", help="skip the 16M-point mixed-vs-FP64 case")
@@ -139,7 +140,8 @@ def timed(K, prob, precision, steps, warmup, graph=False, tuning=None, copy="vie
     device) and mean HIP-event phase times."""
     ev = K.GpuEvaluator(prob, precision=precision, graph=graph)
     if tuning:
-        ev.set_tuning(tuning["p2p_form"], tuning["hadamard_fb"], tuning["hadamard_form"], tuning["overlap"])
+        ev.set_tuning(tuning["p2p_form"], tuning["hadamard_fb"], tuning["hadamard_form"], tuning["overlap"],
+                      tuning.get("up_form", 0), tuning.get("down_form", 0))
     for _ in range(warmup):
         ev.evaluate(copy=copy)
     phases = []
@@ -331,7 +333,7 @@ def tuned_nonuniform(K, steps, warmup, nv):
     names = ("p2p_static", "p2p_persistent", "p2p_static64", "hadamard_valu_fb4", "hadamard_valu_fb8",
              "hadamard_valu_fb16", "hadamard_mfma_fb4", "hadamard_mfma_fb8", "hadamard_mfma_fb16",
              "layout0_u_last", "layout1_u_second_stream", "p2p_mixed_split", "layout2_first_chunk_early",
-             "layout3_two_chunks_early")
+             "layout3_two_chunks_early", "up_gemm", "up_fused", "down_gemm", "down_fused")
     out.update({"default": {"mpoints_s": rate(n, ms0), "ms_per_step": ms0},
                 "tuned": {"mpoints_s": rate(n, ms1), "ms_per_step": ms1, "choice": best, "sampled_rel_err": err,
                           "eager_phases_ms": {k: round(v, 4) for k, v in avg.items()}},
@@ -397,9 +399,9 @@ def main():
                                                         "the copy (the U-list as one launch)"}
     if args.only in (None, "np6"):
         np6 = plain_fp64_np6(K, args.steps, args.warmup, args.nv, not args.no_gpu_tree)
-    if args.only is None and not args.no_large:
+    if args.only in (None, "large") and not args.no_large:
         large = large_mixed(K, max(3, args.steps // 4), 2, args.nv)
-    if args.only is None and not args.no_tuned:
+    if args.only in (None, "nonuniform") and not args.no_tuned:
         tuned = tuned_nonuniform(K, max(5, args.steps // 2), args.warmup, args.nv)
     if args.only in (None, "headline") and not args.no_cpu_baseline:
         threads = args.cpu_threads or int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))
@@ -413,12 +415,22 @@ def main():
                               "included", "n_points": 4194304, "np": 4, "pts_per_box": mt["q"],
                   "precision": mt["precision"]}
         dtype = "f32+bf16+f64" if mt["precision"] == "mixed" else "f64"
-    else:
+    elif np6 is not None:
         value, ms_step = np6["mpoints_s"], np6["ms_per_step"]
         config = {"workload": np6["workload"] + ", copy back included", "n_points": 4194304, "np": 6,
                   "pts_per_box": 512, "precision": "fp64"}
         dtype = "f64"
         roofline = np6["roofline"]
+    elif tuned is not None:
+        value, ms_step = tuned["tuned"]["mpoints_s"], tuned["tuned"]["ms_per_step"]
+        config = {"workload": tuned["workload"] + ", copy back included", "n_points": 1048576, "np": 4,
+                  "pts_per_box": 128, "precision": "fp64"}
+        dtype = "f64"
+    else:
+        value, ms_step = large["np4"]["mixed"]["mpoints_s"], large["np4"]["mixed"]["ms_per_step"]
+        config = {"workload": large["workload"] + ", NP=4, mixed, copy back included", "n_points": 16777216,
+                  "np": 4, "pts_per_box": 256, "precision": "mixed"}
+        dtype = "f32+bf16+f64"
     out = {
         "metric": METRIC,
         "value": value,

@@ -76,8 +76,19 @@ typedef struct kifmm_gpu_tuning {
                         chunks; 1 all of them on the handle's second stream from the
                         start, alongside the far field (initially 1 with KIFMM_GPU_OVERLAP);
                         2, 3 the first one or two chunks there, the rest last */
-  int reserved[4];
+  int up_form;       /* KIFMM_LEVEL_*: how each level's M2M (UE2UC stack, then UC2UE) runs */
+  int down_form;     /* KIFMM_LEVEL_*: how each level's L2L + DC2DE runs */
+  int reserved[2];
 } kifmm_gpu_tuning;
+enum {
+  KIFMM_LEVEL_AUTO = 0,  /* default: column GEMMs for M2M; for L2L + DC2DE the fused kernel at
+                            NP <= 4 and column GEMMs at NP = 6 (the faster form measured on
+                            uniform and clustered inputs) */
+  KIFMM_LEVEL_GEMM = 1,  /* two column GEMMs per level (K split over blocks when the level has
+                            few columns, then a reduction launch) */
+  KIFMM_LEVEL_FUSED = 2  /* one kernel per level: both operator applies of 16 columns per
+                            block, the intermediate vectors kept in LDS */
+};
 enum {
   KIFMM_HAD_VALU = 0, /* a lane per parent, two waves per 64 parents */
   KIFMM_HAD_MFMA = 1  /* 16 parents per 16x16x4 MFMA tile, four waves per 64 parents (default) */
@@ -209,13 +220,15 @@ int kifmm_gpu_copy_intermediate(const kifmm_gpu* g, int which, double* out, long
 int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t);
 int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
 /* Time every U-list form of the handle's path on the U phase, every
- * Hadamard form and block size on the V phase, then both stream layouts on
- * whole (eager) evaluations with the copy back of this handle's own input,
- * reps timed runs each after one warm-up, keep the fastest of each (minimum
- * over runs) and set it.  best (may be NULL) receives the choice; ms (may
- * be NULL, n_ms entries, up to 14) receives the times in the order STATIC,
- * PERSISTENT, STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16, layout 0,
- * layout 1, SPLIT, layout 2, layout 3; -1 for a candidate not timed. */
+ * Hadamard form and block size on the V phase, both level forms on the up
+ * and on the down phase, then the stream layouts on whole (eager)
+ * evaluations with the copy back of this handle's own input, reps timed
+ * runs each after one warm-up, keep the fastest of each (minimum over runs)
+ * and set it.  best (may be NULL) receives the choice; ms (may be NULL, n_ms
+ * entries, up to 18) receives the times in the order STATIC, PERSISTENT,
+ * STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16, layout 0, layout 1, SPLIT,
+ * layout 2, layout 3, up as column GEMMs, up fused, down as column GEMMs,
+ * down fused; -1 for a candidate not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */

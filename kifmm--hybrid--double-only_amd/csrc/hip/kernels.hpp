@@ -1364,11 +1364,8 @@ __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items
  * on gfx950 by test_mfma_f64_operand_layout): A[i][k] in lane i + 16k,
  * B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4), element i/4.
  * With ws != nullptr the unscaled partial product of K-slice blockIdx.z
- * goes to ws[(z * ncols + j) * ldw + row]; the last of the nsplit blocks of
- * an output tile to finish (a device-scope counter per tile, tile_cnt,
- * which that block resets to 0) then adds the partials in slice order and
- * writes the tile, so a split GEMM is one launch and its sums do not depend
- * on which block finishes last. */
+ * goes to ws[(z * ncols + j) * ldw + row], which k_gemm_reduce then adds in
+ * slice order. */
 constexpr int GEMM_KC = 16;
 template <int RT, int CT>
 __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* __restrict__ A, int lda,
@@ -1377,8 +1374,7 @@ __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* _
                                                    const int* __restrict__ sexp,
                                                    const double* __restrict__ in,
                                                    double* __restrict__ out, int accumulate,
-                                                   int ksplit, double* __restrict__ ws, int ldw,
-                                                   int* __restrict__ tile_cnt) {
+                                                   int ksplit, double* __restrict__ ws, int ldw) {
   constexpr int ROWS = 16 * RT, NC = 64 * CT;
   /* row strides chosen so the 4 k-rows a wave reads hit disjoint bank halves */
   constexpr int LA = ROWS + (((16 - ROWS) % 32) + 32) % 32;
@@ -1455,26 +1451,26 @@ __global__ __launch_bounds__(256) void k_gemm_cols(int M, int K, const double* _
       }
     }
   }
-  if (!ws) return;
-  /* split K: the last block of this output tile adds the partials */
-  __shared__ int last;
-  __threadfence(); /* this block's partials are visible device-wide */
-  __syncthreads();
-  const int tile = blockIdx.y * gridDim.x + blockIdx.x;
-  if (tid == 0) last = atomicAdd(tile_cnt + tile, 1) == (int)gridDim.z - 1;
-  __syncthreads();
-  if (!last) return;
-  __threadfence(); /* and the other blocks' partials are visible here */
-  for (int e = tid; e < NC * ROWS; e += 256) {
-    const int col = col0 + e / ROWS, row = row0 + e % ROWS;
-    if (col >= ncols || row >= M) continue;
+}
+
+/* Sums the nsplit partial products of a split-K GEMM in split order and
+ * applies scale / accumulate as k_gemm_cols does.  A launch of its own: a
+ * last-block reduction inside the GEMM needs a device-scope fence per block,
+ * which on gfx950 writes back the XCD's L2 and cost more than this launch. */
+static __global__ __launch_bounds__(256) void k_gemm_reduce(int M, int ncols, int nsplit,
+                                                            const double* __restrict__ ws, int ldw,
+                                                            const int* __restrict__ out_off,
+                                                            const int* __restrict__ sexp, double* __restrict__ out,
+                                                            int accumulate) {
+  const size_t n = (size_t)ncols * M;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const int col = (int)(e / M), row = (int)(e % M);
     double v = 0.0;
-    for (int z = 0; z < (int)gridDim.z; z++) v += ws[((size_t)z * ncols + col) * ldw + row];
+    for (int z = 0; z < nsplit; z++) v += ws[((size_t)z * ncols + col) * ldw + row];
     double* o = out + out_off[col] + row;
     const double t = ldexp(1.0, sexp[col]) * v;
     *o = accumulate ? *o + t : t;
   }
-  if (tid == 0) tile_cnt[tile] = 0; /* ready for the next split launch on this stream */
 }
 
 /* One level of the downward pass with its two operator applies fused
@@ -1683,18 +1679,18 @@ inline GemmShape gemm_shape(int ncols, int M, int K) {
   return s;
 }
 
-/* Launches k_gemm_cols; when K is split, ws must hold nsplit * ncols * ldw
- * doubles and tile_cnt col_blocks * row_groups zeroed ints (left zeroed). */
+/* Launches k_gemm_cols (and k_gemm_reduce when K is split; ws must then
+ * hold nsplit * ncols * ldw doubles). */
 inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, int K, const double* A, int lda,
                                    int ncols, const int* in_off, const int* out_off, const int* sexp,
-                                   const double* in, double* out, int accumulate, double* ws, int* tile_cnt) {
+                                   const double* in, double* out, int accumulate, double* ws) {
   const dim3 grid(sh.col_blocks, sh.row_groups, sh.nsplit);
   double* w = sh.nsplit > 1 ? ws : nullptr;
   const int ks = sh.nsplit > 1 ? sh.ksplit : K;
 #define KIFMM_GEMM_CASE(R, C)                                                                                  \
   if (sh.rt == R && sh.ct == C)                                                                                \
     hipLaunchKernelGGL((k_gemm_cols<R, C>), grid, dim3(256), 0, st, M, K, A, lda, ncols, in_off, out_off, sexp, \
-                       in, out, accumulate, ks, w, sh.ldw, tile_cnt);
+                       in, out, accumulate, ks, w, sh.ldw);
   KIFMM_GEMM_CASE(2, 2)
   else KIFMM_GEMM_CASE(4, 2)
   else KIFMM_GEMM_CASE(4, 1)
@@ -1703,6 +1699,11 @@ inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, i
   else KIFMM_GEMM_CASE(10, 1)
   else return hipErrorInvalidValue;
 #undef KIFMM_GEMM_CASE
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess || sh.nsplit == 1) return e;
+  const size_t n = (size_t)ncols * M;
+  hipLaunchKernelGGL(k_gemm_reduce, dim3((int)min((size_t)2048, (n + 255) / 256)), dim3(256), 0, st, M, ncols,
+                     sh.nsplit, ws, sh.ldw, out_off, sexp, out, accumulate);
   return hipGetLastError();
 }
 

@@ -105,6 +105,7 @@ struct kifmm_gpu {
   int p2p_form = KIFMM_P2P_STATIC;
   int had_fb = 8;
   int had_form = KIFMM_HAD_MFMA;
+  int up_form = KIFMM_LEVEL_AUTO, down_form = KIFMM_LEVEL_AUTO;
   /* mixed path: per target, the U-list sources closer than MIX_CLOSE h */
   int* cptr = nullptr; int* cidx = nullptr;
   int close_listed = 0;
@@ -127,8 +128,6 @@ struct kifmm_gpu {
   size_t chk_cols = 0;
   double* ws = nullptr;
   size_t ws_len = 0;
-  int* tile_cnt = nullptr; /* split-K GEMMs: per output tile, the slices done (0 between launches) */
-  size_t tile_cnt_len = 0;
   kifmm_work_counts counts{};
   /* wall ms of kifmm_gpu_create: host plan, upload, close-pair search,
    * other device set-up (translation spectra, Gram records), whole call */
@@ -310,7 +309,7 @@ struct Plan {
   HostGemm s2m_gemm;
   std::vector<HostLevel> lv;
   size_t chk_cols = 0;
-  size_t ws_len = 0, tile_cnt_len = 0;
+  size_t ws_len = 0;
   std::vector<int> vblocks, vslot, hnbr, hslot, wleaves, xnodes, l2t;
   std::vector<int4> witems, xitems, wred, xred; /* two int4 per item */
   int wx_slots = 0, w_slots = 0, x_slots = 0;     /* partial slots of split W / X lists */
@@ -490,10 +489,7 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
     auto need = [&](size_t ncols, int M, int K) {
       if (!ncols) return;
       const GemmShape sh = gemm_shape((int)ncols, M, K);
-      if (sh.nsplit > 1) {
-        P.ws_len = std::max(P.ws_len, (size_t)sh.nsplit * ncols * sh.ldw);
-        P.tile_cnt_len = std::max(P.tile_cnt_len, (size_t)sh.col_blocks * sh.row_groups);
-      }
+      if (sh.nsplit > 1) P.ws_len = std::max(P.ws_len, (size_t)sh.nsplit * ncols * sh.ldw);
     };
     need(P.s2m_gemm.in.size(), ne, lduc);
     for (const HostLevel& L : P.lv) {
@@ -855,8 +851,6 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
   TRY(dev_zero(g, &g->chk, std::max<size_t>(P.chk_cols, 1) * lduc));
   g->ws_len = P.ws_len;
   TRY(dev_zero(g, &g->ws, std::max<size_t>(P.ws_len, 1)));
-  g->tile_cnt_len = P.tile_cnt_len;
-  TRY(dev_zero(g, &g->tile_cnt, std::max<size_t>(P.tile_cnt_len, 1)));
   g->n_vblocks = (int)P.vblocks.size();
   g->n_hpar = (int)P.hslot.size();
   TRY(dev_upload(g, &g->vblocks, P.vblocks.data(), P.vblocks.size()));
@@ -1012,12 +1006,8 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
     set_error("internal: GEMM workspace too small");
     return KIFMM_EINVAL;
   }
-  if (sh.nsplit > 1 && (size_t)sh.col_blocks * sh.row_groups > g->tile_cnt_len) {
-    set_error("internal: GEMM tile counters too few");
-    return KIFMM_EINVAL;
-  }
   const hipError_t e = launch_gemm_cols(g->stream, sh, M, K, A, lda, L.n, L.in_off, L.out_off, L.sexp, in, out,
-                                        accumulate, g->ws, g->tile_cnt);
+                                        accumulate, g->ws);
   if (e != hipSuccess) {
     set_error(std::string("GEMM launch: ") + hipGetErrorString(e));
     return KIFMM_EDEVICE;
@@ -1025,12 +1015,18 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
   return KIFMM_OK;
 }
 
-/* M2M of the parents at depth d (UE2UC stack, then UC2UE), fused.  A block
- * runs every k step of both operators for its 16 columns, so the fused form
- * is used where that stays short (NP <= 4) or where the level has enough
- * columns to fill the chip (>= 2048); small NP = 6 levels keep the split-K
- * column GEMMs. */
-bool fused_m2m(const kifmm_gpu* g, int ncols) { return g->np <= 4 || ncols >= 2048; }
+/* Per level, M2M (UE2UC stack, then UC2UE) and L2L + DC2DE run either as
+ * two column GEMMs or as one fused kernel.  A fused block runs every k step
+ * of both operators for its 16 columns with 4 waves: fine for the square
+ * n_e x n_e operators of the downward pass at NP <= 4, but the UE2UC stack
+ * (K = 8 n_e) is a long serial chain per block, and the column GEMMs (64
+ * columns a block, K split over blocks on short levels) are faster for M2M
+ * at every NP and for the NP = 6 downward pass.  That is the automatic
+ * choice; KIFMM_LEVEL_GEMM / _FUSED force one form. */
+bool fused_up(const kifmm_gpu* g) { return g->up_form == KIFMM_LEVEL_FUSED; }
+bool fused_down(const kifmm_gpu* g) {
+  return g->down_form == KIFMM_LEVEL_FUSED || (g->down_form == KIFMM_LEVEL_AUTO && g->np <= 4);
+}
 
 int m2m(kifmm_gpu* g, const GemmList& A, const GemmList& B, int d) {
   const dim3 grid((A.n + 15) / 16);
@@ -1295,7 +1291,7 @@ int enqueue_far(kifmm_gpu* g, unsigned mask) {
   }
   for (int d = g->max_depth - 1; d >= 0 && (mask & KIFMM_PHASE_UP); d--) {
     if (g->lv[d].m2m_a.n == 0) continue;
-    if (fused_m2m(g, g->lv[d].m2m_a.n)) {
+    if (fused_up(g)) {
       TRY(m2m(g, g->lv[d].m2m_a, g->lv[d].m2m_b, d));
     } else {
       TRY(gemm(g, g->lv[d].m2m_a, g->n_uc, 8 * lde, g->UE2UC, g->lda_uc, g->uden, g->chk, 0));
@@ -1325,10 +1321,12 @@ int enqueue_far(kifmm_gpu* g, unsigned mask) {
   /* down: DC2DE at depth 2, then per level the fused L2L + DC2DE
    * (k_l2l_dc2de over the parents at depth d - 1), then L2T */
   for (int d = 2; d <= g->max_depth && (mask & KIFMM_PHASE_DOWN); d++) {
-    if (d >= 3)
+    if (d >= 3 && fused_down(g)) {
       TRY(l2l_dc2de(g, g->lv[d - 1].l2l, d));
-    else
-      TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
+      continue;
+    }
+    if (d >= 3) TRY(gemm(g, g->lv[d - 1].l2l, 8 * lde, lde, g->DE2DC, g->lda_l2l, g->dden, g->dchk, 1));
+    TRY(gemm(g, g->lv[d].dc2de, g->n_e, lde, g->DC2DE, g->lda_e, g->dchk, g->dden, 0));
   }
   if ((mask & KIFMM_PHASE_DOWN) && g->n_l2t > 0) {
     if (g->precision == KIFMM_PREC_MIXED)
@@ -1790,9 +1788,11 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
   if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_SPLIT ||
       (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
       (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA) ||
-      t->overlap < 0 || t->overlap > 3) {
+      t->overlap < 0 || t->overlap > 3 || t->up_form < KIFMM_LEVEL_AUTO || t->up_form > KIFMM_LEVEL_FUSED ||
+      t->down_form < KIFMM_LEVEL_AUTO || t->down_form > KIFMM_LEVEL_FUSED) {
     set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16, "
-              "hadamard_form a KIFMM_HAD_* value and overlap 0 to 3");
+              "hadamard_form a KIFMM_HAD_* value, overlap 0 to 3 and up_form / down_form KIFMM_LEVEL_* "
+              "values");
     return KIFMM_EINVAL;
   }
   if (g->precision == KIFMM_PREC_MIXED && t->p2p_form != KIFMM_P2P_STATIC &&
@@ -1806,12 +1806,14 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
     return KIFMM_EUNSUP;
   }
   if (t->p2p_form == g->p2p_form && t->hadamard_fb == g->had_fb && t->hadamard_form == g->had_form &&
-      t->overlap == g->overlap)
+      t->overlap == g->overlap && t->up_form == g->up_form && t->down_form == g->down_form)
     return KIFMM_OK;
   g->p2p_form = t->p2p_form;
   g->had_fb = t->hadamard_fb;
   g->had_form = t->hadamard_form;
   g->overlap = t->overlap;
+  g->up_form = t->up_form;
+  g->down_form = t->down_form;
   for (int k = 0; k < 2; k++) {
     if (g->gexec[k]) (void)hipGraphExecDestroy(g->gexec[k]);
     if (g->graph[k]) (void)hipGraphDestroy(g->graph[k]);
@@ -1831,6 +1833,8 @@ int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t) {
   t->hadamard_fb = g->had_fb;
   t->hadamard_form = g->had_form;
   t->overlap = g->overlap;
+  t->up_form = g->up_form;
+  t->down_form = g->down_form;
   return KIFMM_OK;
 }
 
@@ -1839,19 +1843,25 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[14] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+  double t[18];
+  std::fill(t, t + 18, -1.0);
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
+  /* the fastest of reps timed runs after one warm-up: the U-list, the
+   * Hadamard, or the up and the down phase (out[0], out[1]) */
   auto time_phase = [&](unsigned phase, double* out) -> int {
     kifmm_phase_times pt{};
-    double m = 0.0;
+    double m[2] = {0.0, 0.0};
     for (int r = 0; r <= reps; r++) {
       const int rc = evaluate(g, phase, nullptr, &pt);
       if (rc != KIFMM_OK) return rc;
-      const double v = phase == KIFMM_PHASE_U ? pt.ulist : pt.v_hadamard;
-      if (r == 1 || (r > 1 && v < m)) m = v; /* run 0 warms up */
+      const double v[2] = {phase == KIFMM_PHASE_U ? pt.ulist : phase == KIFMM_PHASE_V ? pt.v_hadamard : pt.up,
+                           pt.down};
+      for (int k = 0; k < 2; k++)
+        if (r == 1 || (r > 1 && v[k] < m[k])) m[k] = v[k]; /* run 0 warms up */
     }
-    *out = m;
+    out[0] = m[0];
+    if (phase == (KIFMM_PHASE_UP | KIFMM_PHASE_DOWN)) out[1] = m[1];
     return KIFMM_OK;
   };
   kifmm_gpu_tuning c = cur;
@@ -1885,6 +1895,18 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   }
   c.hadamard_form = best_v / 3;
   c.hadamard_fb = fbs[best_v % 3];
+  /* up and down: the column GEMMs against the fused kernels, each phase on
+   * its own (slots 14, 15: up; 16, 17: down) */
+  for (int k = 0; k < 2; k++) {
+    c.up_form = c.down_form = k == 0 ? KIFMM_LEVEL_GEMM : KIFMM_LEVEL_FUSED;
+    TRY(kifmm_gpu_set_tuning(g, &c));
+    double ud[2];
+    TRY(time_phase(KIFMM_PHASE_UP | KIFMM_PHASE_DOWN, ud));
+    t[14 + k] = ud[0];
+    t[16 + k] = ud[1];
+  }
+  c.up_form = t[15] < t[14] ? KIFMM_LEVEL_FUSED : KIFMM_LEVEL_GEMM;
+  c.down_form = t[17] < t[16] ? KIFMM_LEVEL_FUSED : KIFMM_LEVEL_GEMM;
   /* stream layout, on whole evaluations timed eagerly (a graph handle
    * would otherwise capture here) */
   const int use_graph = g->use_graph;
@@ -1908,7 +1930,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 14); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 18); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

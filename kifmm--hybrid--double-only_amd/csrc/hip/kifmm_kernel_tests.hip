@@ -301,12 +301,11 @@ int kifmm_test_gemm(int M, int K, const double* A, int ncols, const double* Bm, 
   /* the evaluator's shape rule and launcher */
   const GemmShape sh = gemm_shape(ncols, M, Kp);
   double* dws = sh.nsplit > 1 ? B.up<double>(nullptr, (size_t)sh.nsplit * ncols * sh.ldw) : nullptr;
-  int* dcnt = sh.nsplit > 1 ? B.up<int>(nullptr, (size_t)sh.col_blocks * sh.row_groups) : nullptr;
-  if (sh.nsplit > 1 && (!dws || !dcnt)) {
+  if (sh.nsplit > 1 && !dws) {
     set_error("device allocation failed");
     return KIFMM_ENOMEM;
   }
-  if (launch_gemm_cols(0, sh, M, Kp, dA, lda, ncols, din, dout, dse, dB, dC, accumulate, dws, dcnt) != hipSuccess) {
+  if (launch_gemm_cols(0, sh, M, Kp, dA, lda, ncols, din, dout, dse, dB, dC, accumulate, dws) != hipSuccess) {
     set_error("GEMM launch failed");
     return KIFMM_EDEVICE;
   }

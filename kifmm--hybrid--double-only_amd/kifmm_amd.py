@@ -28,6 +28,7 @@ GPU_NO_CLOSE_LIST = 2
 GPU_MIXED_INLINE = GPU_NO_CLOSE_LIST  # earlier name of the same flag
 GPU_OVERLAP = 4
 PHASE_UP, PHASE_U, PHASE_V, PHASE_W, PHASE_X, PHASE_DOWN, PHASE_ALL = 1, 2, 4, 8, 16, 32, 63
+LEVEL_AUTO, LEVEL_GEMM, LEVEL_FUSED = 0, 1, 2
 
 
 class KifmmError(RuntimeError):
@@ -73,7 +74,7 @@ class GpuOptions(C.Structure):
 
 class GpuTuning(C.Structure):
     _fields_ = [("p2p_form", C.c_int), ("hadamard_fb", C.c_int), ("hadamard_form", C.c_int),
-                ("overlap", C.c_int), ("reserved", C.c_int * 4)]
+                ("overlap", C.c_int), ("up_form", C.c_int), ("down_form", C.c_int), ("reserved", C.c_int * 2)]
 
 
 P2P_STATIC, P2P_PERSISTENT, P2P_STATIC64, P2P_SPLIT = 0, 1, 2, 3
@@ -448,13 +449,17 @@ class GpuEvaluator:
         t = GpuTuning()
         _check(lib.kifmm_gpu_get_tuning(self._h, C.byref(t)))
         return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
-                "overlap": t.overlap}
+                "overlap": t.overlap, "up_form": t.up_form, "down_form": t.down_form}
 
-    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8, hadamard_form=HAD_MFMA, overlap=None):
-        """overlap=None keeps the handle's current stream layout."""
-        if overlap is None:
-            overlap = self.tuning()["overlap"]
-        t = GpuTuning(p2p_form, hadamard_fb, hadamard_form, int(overlap))
+    def set_tuning(self, p2p_form=P2P_STATIC, hadamard_fb=8, hadamard_form=HAD_MFMA, overlap=None, up_form=None,
+                   down_form=None):
+        """overlap / up_form / down_form None keep the handle's current
+        stream layout / level forms."""
+        cur = self.tuning()
+        overlap = cur["overlap"] if overlap is None else overlap
+        up_form = cur["up_form"] if up_form is None else up_form
+        down_form = cur["down_form"] if down_form is None else down_form
+        t = GpuTuning(p2p_form, hadamard_fb, hadamard_form, int(overlap), int(up_form), int(down_form))
         _check(lib.kifmm_gpu_set_tuning(self._h, C.byref(t)))
 
     def autotune(self, reps=3):
@@ -462,13 +467,14 @@ class GpuEvaluator:
         (choice, ms) with ms for STATIC, PERSISTENT, STATIC64, then the
         Hadamard on the VALU with 4, 8, 16 frequencies per block and on MFMA
         with 4, 8, 16, then whole evaluations with the copy back in stream
-        layouts 0 and 1, then SPLIT (mixed path), then layouts 2 and 3; -1
+        layouts 0 and 1, then SPLIT (mixed path), then layouts 2 and 3, then
+        up as column GEMMs and fused, down as column GEMMs and fused; -1
         where not timed."""
         t = GpuTuning()
-        ms = np.zeros(14)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 14))
+        ms = np.zeros(18)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 18))
         return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
-                "overlap": t.overlap}, ms
+                "overlap": t.overlap, "up_form": t.up_form, "down_form": t.down_form}, ms
 
     def work_counts(self):
         w = WorkCounts()

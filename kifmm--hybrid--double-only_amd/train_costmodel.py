@@ -9,7 +9,9 @@ of 8 Gaussian clusters, NP in {3, 4, 6}.  Every input is built at every q in
 U / V / down ms).  The network is trained on the GPU by the HIP kernels
 (kifmm_mlp_train).  Held-out set: 16 further inputs, each swept exhaustively;
 the report gives, per input, the model's choice against the best of the
-sweep and against a fixed q = 64 on the FP64 path.
+sweep and against a fixed q = 64 on the FP64 path, each with its time and
+its sampled relative error against the FP64 direct sum (NV = 1000 targets,
+one sample shared by the configurations of an input).
 
     python train_costmodel.py --out DIR [--train 64] [--held 16]
 
@@ -67,6 +69,28 @@ def sweep(inp, qs, steps, tag=""):
     return rows, pts
 
 
+def sampled_errors(pts, np_, configs, nv):
+    """Sampled relative L2 error of each (q, precision) in configs on one
+    shared sample of nv targets (direct FP64 sums on the GPU)."""
+    out, sample = {}, None
+    for q, prec in configs:
+        if (q, prec) in out:
+            continue
+        prob = K.Problem.from_points(*pts, pts_max=q, np_=np_, device=0)
+        ev = K.GpuEvaluator(prob, precision=prec)
+        pot = prob.scatter(ev.evaluate()[0])
+        if sample is None:
+            err, ids, direct = prob.validate_gpu(ev, pot, nv)
+            sample = (ids, direct)
+        else:
+            f = pot[sample[0]]
+            err = float(np.linalg.norm(sample[1] - f) / np.linalg.norm(f))
+        out[(q, prec)] = float(err)
+        ev.close()
+        del prob
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--out", required=True)
@@ -77,13 +101,20 @@ def main():
     ap.add_argument("--lr", type=float, default=3e-3)
     ap.add_argument("--collect", default=None, help="A:B -- only measure training inputs A..B-1")
     ap.add_argument("--data", nargs="*", default=None, help="train on these collected files")
+    ap.add_argument("--nv", type=int, default=1000, help="targets of the sampled error (0: skip)")
+    ap.add_argument("--report-only", action="store_true",
+                    help="keep the committed weights: only the held-out sweep and its report")
     args = ap.parse_args()
     out = Path(args.out)
     out.mkdir(parents=True, exist_ok=True)
     qs = CM.Q_CHOICES
     t0 = time.time()
     data = []
-    if args.data:
+    if args.report_only:
+        model = CM.CostModel.load()
+        if model is None:
+            raise SystemExit("no committed weights to report on")
+    elif args.data:
         for f in args.data:
             data += json.loads(Path(f).read_text())
     else:
@@ -99,13 +130,14 @@ def main():
         (out / "costmodel_data.json").write_text(json.dumps(data))
         if args.collect is not None:
             return 0
-    X = np.array([r["features"] for r in data])
-    Y = np.array([r["phases_ms"] for r in data])
-    model = CM.CostModel.fit(X, Y, epochs=args.epochs, lr=args.lr)
-    model.meta.update({"train_inputs": args.train, "q_choices": list(qs)})
-    model.save(out / "costmodel_weights.json")
-    print(f"trained on {len(data)} rows: loss {model.meta['first_loss']:.4f} -> {model.meta['final_loss']:.5f}",
-          flush=True)
+    if not args.report_only:
+        X = np.array([r["features"] for r in data])
+        Y = np.array([r["phases_ms"] for r in data])
+        model = CM.CostModel.fit(X, Y, epochs=args.epochs, lr=args.lr)
+        model.meta.update({"train_inputs": args.train, "q_choices": list(qs)})
+        model.save(out / "costmodel_weights.json")
+        print(f"trained on {len(data)} rows: loss {model.meta['first_loss']:.4f} -> "
+              f"{model.meta['final_loss']:.5f}", flush=True)
     tuner = CM.Tuner(model, qs)
     report = []
     for k, inp in enumerate(draw_inputs(args.held, 5000, 11)):
@@ -121,25 +153,38 @@ def main():
         # Tuner(verify=2): the two best-ranked candidates timed, the faster kept
         top2 = np.argsort(pred_all, kind="stable")[:2]
         ver = rows[int(top2[np.argmin(meas_all[top2])])]
+        err = {}
+        if args.nv > 0:
+            err = sampled_errors(pts, inp[2], [(64, K.PREC_FP64), (q, prec), (ver["q"], ver["prec"]),
+                                               (best["q"], best["prec"])], args.nv)
+        e = lambda q_, p_: err.get((q_, p_))
         rec = {"n": inp[0], "dist": inp[1], "np": inp[2], "seed": inp[3],
-               "chosen": {"q": q, "prec": prec, "ms": chosen["wall_ms"], "predicted_ms": pred},
-               "best": {"q": best["q"], "prec": best["prec"], "ms": best["wall_ms"]},
+               "chosen": {"q": q, "prec": prec, "ms": chosen["wall_ms"], "predicted_ms": pred,
+                          "sampled_rel_err": e(q, prec)},
+               "best": {"q": best["q"], "prec": best["prec"], "ms": best["wall_ms"],
+                        "sampled_rel_err": e(best["q"], best["prec"])},
                "default_q64_fp64_ms": default["wall_ms"],
+               "default_q64_fp64_sampled_rel_err": e(64, K.PREC_FP64),
                "chosen_over_best": chosen["wall_ms"] / best["wall_ms"],
-               "verify2": {"q": ver["q"], "prec": ver["prec"], "ms": ver["wall_ms"]},
+               "verify2": {"q": ver["q"], "prec": ver["prec"], "ms": ver["wall_ms"],
+                           "sampled_rel_err": e(ver["q"], ver["prec"])},
                "verify2_over_best": ver["wall_ms"] / best["wall_ms"],
                "gain_over_default": default["wall_ms"] / chosen["wall_ms"],
                "median_abs_log_error": float(np.median(np.abs(np.log(pred_all / meas_all))))}
         report.append(rec)
         print(f"held {k + 1}/{args.held} n={inp[0]} {inp[1]} NP={inp[2]}: model q={q} prec={prec} "
               f"{chosen['wall_ms']:.2f} ms, best q={best['q']} prec={best['prec']} {best['wall_ms']:.2f} ms, "
-              f"q=64 FP64 {default['wall_ms']:.2f} ms", flush=True)
+              f"q=64 FP64 {default['wall_ms']:.2f} ms; sampled errors model {e(q, prec)} best "
+              f"{e(best['q'], best['prec'])}", flush=True)
     summ = {"held_out": report,
             "geomean_chosen_over_best": float(np.exp(np.mean(np.log([r["chosen_over_best"] for r in report])))),
             "geomean_gain_over_default": float(np.exp(np.mean(np.log([r["gain_over_default"] for r in report])))),
             "exact_pick_fraction": float(np.mean([r["chosen_over_best"] <= 1.0 + 1e-9 for r in report])),
             "geomean_verify2_over_best": float(np.exp(np.mean(np.log([r["verify2_over_best"] for r in report])))),
             "exact_pick_fraction_verify2": float(np.mean([r["verify2_over_best"] <= 1.0 + 1e-9 for r in report])),
+            "worst_chosen_over_best": float(max(r["chosen_over_best"] for r in report)),
+            "max_chosen_sampled_rel_err": {str(p): max((r["chosen"]["sampled_rel_err"] or 0.0) for r in report
+                                                       if r["np"] == p) for p in (3, 4, 6)},
             "model": model.meta}
     (out / "costmodel_report.json").write_text(json.dumps(summ, indent=1))
     print(json.dumps({k: v for k, v in summ.items() if k != "held_out"}), flush=True)

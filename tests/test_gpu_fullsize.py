@@ -55,7 +55,8 @@ def test_ellipsoid_1m_np4_tuned_graph_matches_oracle():
     best, _ = tuner.autotune(reps=2)
     del tuner
     ev = K.GpuEvaluator(prob, graph=True)
-    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"], best["overlap"])
+    ev.set_tuning(best["p2p_form"], best["hadamard_fb"], best["hadamard_form"], best["overlap"], best["up_form"],
+                best["down_form"])
     got = prob.scatter(ev.evaluate()[0])
     again = prob.scatter(ev.evaluate()[0])
     assert np.array_equal(got, again)
