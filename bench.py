@@ -46,6 +46,21 @@ P2P_FLOPS_PER_PAIR = 11.0  # the reference's convention, fmm/driver.cc:60
 FP64_PEAK_TFLOPS = 78.6    # MI355X FP64 (vector and dense matrix)
 FP32_PEAK_TFLOPS = 157.3   # MI355X FP32 vector
 HBM_PEAK_GBS = 8000.0
+# Instruction-issue ceilings of the U-list kernels' inner loops (gfx950
+# assembly of this build; packed FP32 is off, transcendentals are quarter
+# rate, and MFMA issue adds to VALU issue rather than overlapping it):
+#   mixed: per 1024 pairs 4 v_mfma_f32_16x16x32_bf16 (16 cycles each),
+#          16 v_rsq_f32 (16) and 16 v_fmac_f32 (4): 384 cycles per SIMD;
+#   FP64 Gram form: per 64 pairs 8 full-rate FP64 instructions (4 cycles each)
+#          and one v_rsq_f64 (16): 48 cycles per SIMD.
+SIMDS, CLOCK_GHZ = 1024, 2.4
+ISSUE_PAIRS_PER_CYCLE = {"mixed": 1024 / 384, "fp64": 64 / 48}
+
+
+def issue_bound(kind, achieved_tflops):
+    t = P2P_FLOPS_PER_PAIR * ISSUE_PAIRS_PER_CYCLE[kind] * SIMDS * CLOCK_GHZ * 1e9 / 1e12
+    return {"tflops": t, "frac": achieved_tflops / t, "pairs_per_cycle_per_simd": ISSUE_PAIRS_PER_CYCLE[kind],
+            "model": "inner-loop instruction issue at 2.4 GHz on 1024 SIMDs (quarter-rate rsq, no packed FP32)"}
 METRIC = "Mpoints/s per FMM eval (4M uniform), mixed vs FP64 GPU at matched sampled rel err"
 
 
@@ -306,7 +321,7 @@ def plain_fp64_np6(K, steps, warmup, nv, gpu_tree):
                      "unit": "TFLOP/s", "frac": achieved / FP64_PEAK_TFLOPS,
                      "traffic": read_traffic("p2p_ulist_uniform4M_np6_counters.json"),
                      "algorithmic": f"{P2P_FLOPS_PER_PAIR:g} flop/pair x {wc['u_pairs']:.4g} U-list pairs per launch",
-                     "kernel_ms": p2p_ms}})
+                     "kernel_ms": p2p_ms, "issue_bound": issue_bound("fp64", achieved)}})
     return out
 
 
@@ -396,7 +411,8 @@ def main():
                     "traffic": read_traffic("p2p_ulist_mixed_uniform4M_np4_counters.json"),
                     "algorithmic": f"{P2P_FLOPS_PER_PAIR:g} flop/pair x {wc['u_pairs']:.4g} U-list pairs per launch",
                     "kernel_ms": p2p_ms, "measured_on": "the model-tuned configuration evaluated eagerly without "
-                                                        "the copy (the U-list as one launch)"}
+                                                        "the copy (the U-list as one launch)",
+                    "issue_bound": issue_bound("mixed" if prec == K.PREC_MIXED else "fp64", achieved)}
     if args.only in (None, "np6"):
         np6 = plain_fp64_np6(K, args.steps, args.warmup, args.nv, not args.no_gpu_tree)
     if args.only in (None, "large") and not args.no_large:

@@ -1913,11 +1913,23 @@ __global__ __launch_bounds__(256) void k_trans(const double2* __restrict__ tw, S
  * slot 0.
  *
  * Block = one group of up to 64 parents (lane = parent) at one frequency f
- * (blockIdx.y; uniform, so translation entries are scalar loads).  The
+ * (block-uniform, so translation entries are scalar loads).  The
  * union of the group's neighbour slots is staged in LDS once (216 slots
  * for an aligned 4x4x4 group of parents instead of 64 x 26 reads from L2);
  * wave h computes target children s in [4h, 4h+4). */
 constexpr int HAD_CAP = 232; /* LDS slots per group */
+
+/* Block order of the Hadamard launches (1-D grid of 8 per_xcd blocks, per_xcd
+ * = ceil(n_groups * n_fblocks / 8)): the XCDs take consecutive blocks round
+ * robin, so block b runs on XCD b % 8, and v = (b % 8) per_xcd + b / 8 gives
+ * each XCD one contiguous run of v = frequency block * n_groups + group.  All
+ * the groups of a frequency block then run on one XCD, and the source
+ * spectra of that block, which the groups' neighbour unions share, are
+ * fetched into one L2 instead of eight.  Returns -1 past the last pair. */
+__device__ __forceinline__ int had_block(int n_groups, int n_fblocks, int per_xcd) {
+  const int v = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
+  return v < n_groups * n_fblocks ? v : -1;
+}
 template <int NP, int HALF, typename ST>
 __device__ __forceinline__ void hadamard_half(const ST (*Xs)[9], const unsigned char (*sloc)[64], int lane,
                                               const ST* __restrict__ Tf, typename Spec<ST>::R yr[4],
@@ -1960,7 +1972,7 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
                                                   const int* __restrict__ yslot,   /* Y slot per parent */
                                                   const ST* __restrict__ T,
                                                   const ST* __restrict__ X,
-                                                  ST* __restrict__ Y) {
+                                                  ST* __restrict__ Y, int n_groups, int per_xcd) {
   typedef typename Spec<ST>::R R;
   using G = Grid<NP>;
   constexpr int F = G::F;
@@ -1968,8 +1980,10 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
   constexpr int PER = (HAD_CAP * 8 + 127) / 128;
   __shared__ ST Xs[HAD_CAP][9];
   __shared__ unsigned char sloc[27][64];
-  const int4 g = groups[blockIdx.x];
-  const int f0 = blockIdx.y * FB;
+  const int v = had_block(n_groups, F / FB, per_xcd);
+  if (v < 0) return; /* whole block, before any barrier */
+  const int4 g = groups[v % n_groups];
+  const int f0 = (v / n_groups) * FB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int half = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ne8 = g.w * 8;
@@ -2032,8 +2046,18 @@ __global__ __launch_bounds__(128) void k_hadamard(const int4* __restrict__ group
  * lane reads two translation values (A) and two neighbour spectra (B)
  * from LDS; the group's neighbour-slot union and the frequency's 343
  * translation values are staged per frequency, the next frequency's
- * prefetched into registers meanwhile.  MFMA operand layouts as pinned by
- * test_mfma_f64_operand_layout / test_mfma_f32_operand_layout. */
+ * prefetched into registers meanwhile.  A lane's A operand is Re or Im of a
+ * translation value for the real-part MFMAs and -Im or Re for the
+ * imaginary-part ones, by the lane's row.  In FP32 the translation values
+ * are staged twice, as (Re, Im) in TA and (-Im, Re) in TB, so that the
+ * operand is one LDS read at a lane-fixed offset, with no select or negation
+ * on the VALU, whose issue adds to the 35-cycle FP32 MFMAs' (-15 % at 16M
+ * points, NP 4).  The FP64 form keeps one (Re, Im) copy and selects: its
+ * fully unrolled direction loop prefetches the LDS reads of later directions
+ * at 128 VGPRs, which the 64-cycle FP64 MFMAs need more than the saved VALU
+ * slots (the staged-twice form measured 5 % slower there).  MFMA operand
+ * layouts as pinned by test_mfma_f64_operand_layout /
+ * test_mfma_f32_operand_layout. */
 __device__ __forceinline__ v4d mfma4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
@@ -2056,7 +2080,8 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
                                                        const int* __restrict__ uslots,
                                                        const unsigned char* __restrict__ loc,
                                                        const int* __restrict__ yslot, const ST* __restrict__ T,
-                                                       const ST* __restrict__ X, ST* __restrict__ Y) {
+                                                       const ST* __restrict__ X, ST* __restrict__ Y, int n_groups,
+                                                       int per_xcd) {
   typedef typename Spec<ST>::R R;
   typedef typename Acc4<R>::T V;
   using G = Grid<NP>;
@@ -2065,10 +2090,16 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
   constexpr int PER = (HAD_CAP * 8 + 255) / 256;
   constexpr int PERT = (343 + 255) / 256;
   __shared__ ST Xs[HAD_CAP][9];
-  __shared__ ST Ts[343];
+  constexpr bool F32 = sizeof(R) == 4;
+  /* FP32: TA = (Re, Im) then TB = (-Im, Re), 2 x 343 float2; FP64: Ts = (Re, Im) */
+  __shared__ ST Tsh[F32 ? 2 * 343 : 343];
+  R* const TA = reinterpret_cast<R*>(Tsh);
+  R* const TB = TA + 2 * 343;
   __shared__ unsigned char sloc[27][64];
-  const int4 g = groups[blockIdx.x];
-  const int f0 = blockIdx.y * FB;
+  const int v = had_block(n_groups, F / FB, per_xcd);
+  if (v < 0) return; /* whole block, before any barrier */
+  const int4 g = groups[v % n_groups];
+  const int f0 = (v / n_groups) * FB;
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int ne8 = g.w * 8;
@@ -2105,11 +2136,16 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
    * source children this lane supplies are kk and kk + 4 */
   const int c = j & 7;
   const bool rrow = j < 8;
+  const int sel = rrow ? 0 : 1; /* rows 0-7: Re Y (A = Re T, -Im T); rows 8-15: Im Y (A = Im T, Re T) */
   auto base = [&](int cs) {
     return (3 + ((cs >> 2) & 1) - ((c >> 2) & 1)) + 7 * (3 + ((cs >> 1) & 1) - ((c >> 1) & 1)) +
            49 * (3 + (cs & 1) - (c & 1));
   };
   const int b0 = base(kk), b1 = base(kk + 4);
+  /* FP32: this lane's A operands for source children kk and kk + 4, before
+   * the direction offset */
+  const R* const a0r = TA + 2 * b0 + sel;
+  const R* const a1r = TA + 2 * b1 + sel;
   const int ys = yslot[g.x + min(pj, g.y - 1)];
   for (int fi = 0; fi < FB; fi++) {
     const int f = f0 + fi;
@@ -2122,7 +2158,16 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
 #pragma unroll
     for (int k = 0; k < PERT; k++) {
       const int e = tid + 256 * k;
-      if (e < 343) Ts[e] = pret[k];
+      if (e < 343) {
+        if constexpr (F32) {
+          TA[2 * e] = pret[k].x;
+          TA[2 * e + 1] = pret[k].y;
+          TB[2 * e] = -pret[k].y;
+          TB[2 * e + 1] = pret[k].x;
+        } else {
+          Tsh[e] = pret[k];
+        }
+      }
     }
     __syncthreads();
     if (fi + 1 < FB) fetch(f + 1);
@@ -2130,17 +2175,43 @@ __global__ __launch_bounds__(256) void k_hadamard_mfma(const int4* __restrict__ 
     /* four independent accumulator chains: no MFMA waits on the one just
      * issued */
     V accr0 = {0, 0, 0, 0}, acci0 = {0, 0, 0, 0}, accr1 = {0, 0, 0, 0}, acci1 = {0, 0, 0, 0};
+    if constexpr (!F32) {
 #pragma unroll 2
-    for (int o = 0; o < 27; o++) {
-      if (o == 13) continue; /* the parent itself: its children are all adjacent */
-      const int os = 2 * (o / 9 - 1) + 14 * ((o / 3) % 3 - 1) + 98 * (o % 3 - 1);
-      const ST t0 = Ts[b0 + os], t1 = Ts[b1 + os];
-      const ST* xs = Xs[sloc[o][pj]];
-      const ST x0 = xs[kk], x1 = xs[kk + 4];
-      accr0 = mfma4(rrow ? t0.x : t0.y, x0.x, accr0);
-      acci0 = mfma4(rrow ? -t0.y : t0.x, x0.y, acci0);
-      accr1 = mfma4(rrow ? t1.x : t1.y, x1.x, accr1);
-      acci1 = mfma4(rrow ? -t1.y : t1.x, x1.y, acci1);
+      for (int o = 0; o < 27; o++) {
+        if (o == 13) continue; /* the parent itself: its children are all adjacent */
+        const int os = 2 * (o / 9 - 1) + 14 * ((o / 3) % 3 - 1) + 98 * (o % 3 - 1);
+        const ST t0 = Tsh[b0 + os], t1 = Tsh[b1 + os];
+        const ST* xs = Xs[sloc[o][pj]];
+        const ST x0 = xs[kk], x1 = xs[kk + 4];
+        accr0 = mfma4(rrow ? t0.x : t0.y, x0.x, accr0);
+        acci0 = mfma4(rrow ? -t0.y : t0.x, x0.y, acci0);
+        accr1 = mfma4(rrow ? t1.x : t1.y, x1.x, accr1);
+        acci1 = mfma4(rrow ? -t1.y : t1.x, x1.y, acci1);
+      }
+    } else {
+    /* the 26 directions (ox, oy, oz) in the order o = 9 (ox + 1) + 3 (oy +
+     * 1) + oz + 1; only the z loop is unrolled: unrolling all of them would
+     * hoist the LDS reads of many directions into registers and halve the
+     * occupancy */
+#pragma unroll 1
+    for (int ox = -1; ox <= 1; ox++)
+#pragma unroll 1
+      for (int oy = -1; oy <= 1; oy++) {
+        const int oxy = 9 * (ox + 1) + 3 * (oy + 1);
+        const int osxy = 2 * (2 * ox + 14 * oy); /* in R elements */
+#pragma unroll
+        for (int oz = -1; oz <= 1; oz++) {
+          if (ox == 0 && oy == 0 && oz == 0) continue; /* the parent itself: its children are all adjacent */
+          const ST* xs = Xs[sloc[oxy + oz + 1][pj]];
+          const ST x0 = xs[kk], x1 = xs[kk + 4];
+          const R* const p0 = a0r + osxy + 2 * 98 * oz;
+          const R* const p1 = a1r + osxy + 2 * 98 * oz;
+          accr0 = mfma4(p0[0], x0.x, accr0);
+          acci0 = mfma4(p0[2 * 343], x0.y, acci0);
+          accr1 = mfma4(p1[0], x1.x, accr1);
+          acci1 = mfma4(p1[2 * 343], x1.y, acci1);
+        }
+      }
     }
     const V acc = (accr0 + accr1) + (acci0 + acci1);
     ST* yb = Y + ((size_t)ys * F + f) * 8;

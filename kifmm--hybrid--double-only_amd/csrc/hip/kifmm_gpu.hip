@@ -1086,16 +1086,19 @@ int vlist_phase(kifmm_gpu* g, ST* X, ST* Y, ST* T, hipEvent_t e1, hipEvent_t e2,
   /* the translation spectra T depend only on NP: computed once at create
    * time (trans_setup), so this sub-phase is empty */
   REC(e2);
+  /* XCD-contiguous block order (had_block): 8 per_xcd blocks for the
+   * n_hgroups x F / fb (group, frequency block) pairs */
+  const int had_per_xcd = (int)(((long)g->n_hgroups * (g->F / g->had_fb) + 7) / 8);
   if (g->n_hgroups > 0)
     switch (g->had_fb * 2 + g->had_form) {
 #define KIFMM_HAD_CASE(FBV)                                                                                    \
   case 2 * FBV:                                                                                                \
-    hipLaunchKernelGGL((k_hadamard<NP, ST, FBV>), dim3(g->n_hgroups, g->F / FBV), dim3(128), 0, g->stream,     \
-                       g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);                                   \
+    hipLaunchKernelGGL((k_hadamard<NP, ST, FBV>), dim3(8 * had_per_xcd), dim3(128), 0, g->stream, g->hgroups,  \
+                       g->huslots, g->hloc, g->hslot, T, X, Y, g->n_hgroups, had_per_xcd);                    \
     break;                                                                                                     \
   case 2 * FBV + 1:                                                                                            \
-    hipLaunchKernelGGL((k_hadamard_mfma<NP, ST, FBV>), dim3(g->n_hgroups, g->F / FBV), dim3(256), 0, g->stream, \
-                       g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y);                                   \
+    hipLaunchKernelGGL((k_hadamard_mfma<NP, ST, FBV>), dim3(8 * had_per_xcd), dim3(256), 0, g->stream,         \
+                       g->hgroups, g->huslots, g->hloc, g->hslot, T, X, Y, g->n_hgroups, had_per_xcd);        \
     break;
       KIFMM_HAD_CASE(4)
       KIFMM_HAD_CASE(8)
