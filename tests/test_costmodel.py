@@ -45,3 +45,27 @@ def test_saved_model_roundtrip(tmp_path):
     m.save(f)
     m2 = CM.CostModel.load(f)
     assert np.array_equal(m2.params, m.params) and m2.meta["rows"] == 1
+
+
+def test_committed_report_meets_its_claims():
+    """The held-out report of the committed weights (train_costmodel.py):
+    16 inputs not in the training set, each with the model's pick, the best
+    of the exhaustive sweep and the fixed q = 64 FP64 default, timed, and
+    each pick's sampled error against the FP64 direct sum.  The pick is near
+    the best (geomean within 5 %, never worse than 1.25x), gains over the
+    default, and every sampled error is at the FMM's own level for its NP."""
+    import json
+    from pathlib import Path
+    rep = json.loads((Path(CM.__file__).parent / "costmodel_report.json").read_text())
+    held = rep["held_out"]
+    assert len(held) == 16
+    bound = {3: 5e-4, 4: 1e-4, 6: 1e-6}
+    for r in held:
+        for k in ("chosen", "best", "verify2"):
+            e = r[k]["sampled_rel_err"]
+            assert e is not None and 0 < e <= bound[r["np"]], (r["n"], r["dist"], k, e)
+        assert r["chosen"]["ms"] >= r["best"]["ms"] and r["chosen_over_best"] <= 1.25
+    g = lambda k: float(np.exp(np.mean(np.log([r[k] for r in held]))))
+    assert g("chosen_over_best") <= 1.05 and g("verify2_over_best") <= 1.01
+    assert g("gain_over_default") >= 1.5
+    assert rep["model"]["rows"] > 500 and rep["model"]["final_loss"] < rep["model"]["first_loss"]

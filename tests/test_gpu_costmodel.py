@@ -65,7 +65,9 @@ def test_trained_model_gains_over_default_q():
         _, t_default = CM.phase_times(base, K.PREC_FP64, steps=5)
         print(f"{dist} n={n} NP={np_}: model q={q} prec={prec} {t_model:.3f} ms; default q=64 FP64 {t_default:.3f} ms")
         rows.append(t_default / t_model)
-    assert np.exp(np.mean(np.log(rows))) >= 1.0
+    # the committed report's held-out geomean is 2.08 (16 inputs, least 1.42);
+    # 1.5 leaves room for box-to-box timing spread on these three
+    assert np.exp(np.mean(np.log(rows))) >= 1.5
 
 
 def test_verified_choice_is_one_of_the_best_ranked():
