@@ -1262,10 +1262,15 @@ static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ ite
 /* Mixed-precision S2M and L2T (NP <= 4): the same lane-per-target scheme
  * in FP32, in units of the leaf's half-width h centred on the leaf (the
  * check and equivalent surfaces are unit-box surfaces there, shared by all
- * leaves), summed in FP32 per lane and scaled by 1/h into the FP64 check
- * values or potentials.  Surface points and sources are at least 1.27 h
- * apart, so FP32 differences carry ~1e-7 relative error; the pinv applies
- * that follow stay FP64. */
+ * leaves, rows x, y, z, |p|^2), summed in FP32 per lane and scaled by 1/h
+ * into the FP64 check values or potentials.  Per pair the Gram form
+ *     r^2 = (|p|^2 + |s|^2) + (-2p) . s          (1 add, 3 FMA)
+ * then v_rsq_f32 and the FMA with the weight: 5 VALU instructions and the
+ * rsq, against 7 for coordinate differences.  The surfaces lie at least 2
+ * leaf units (max norm) from the leaf's points, |p| <= 3 sqrt(3) and |s| <=
+ * sqrt(3), so the Gram r^2 carries at most ~12 FP32 roundings of r^2, ~1e-6
+ * relative in 1 / r (the FP32 sum over the leaf's points is of the same
+ * order); the pinv applies that follow stay FP64. */
 /* FP32 leaf-local source records for the mixed S2M: (s - c) / h and w for
  * every source of every source leaf (one block per leaf). */
 static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restrict__ items,
@@ -1284,67 +1289,90 @@ static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restri
 
 /* LPB consecutive leaves per block in the mixed S2M and L2T (1 for leaves
  * of ~100 points and more; 4 for small leaves, whose per-block cost and
- * surface loads are then shared). */
+ * surface loads are then shared).  S2M: lane = check point; the leaf's
+ * sources are staged in LDS as (s, |s|^2) and w, read as broadcasts. */
 template <int LPB>
 __global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items, int n_items,
                                                           const double4* __restrict__ geo,
                                                           const float* __restrict__ surf_uc32, int n_uc,
                                                           int ld_uc_surf, const float4* __restrict__ src32,
                                                           double* __restrict__ chk, int ld_chk) {
+  __shared__ float4 sS[S2M_TILE];
+  __shared__ float sW[S2M_TILE];
   const int j = threadIdx.x, jj = min(j, n_uc - 1);
-  const float tx = surf_uc32[jj], ty = surf_uc32[ld_uc_surf + jj], tz = surf_uc32[2 * ld_uc_surf + jj];
+  const float ax = -2.0f * surf_uc32[jj], ay = -2.0f * surf_uc32[ld_uc_surf + jj],
+              az = -2.0f * surf_uc32[2 * ld_uc_surf + jj], pp = surf_uc32[3 * ld_uc_surf + jj];
   for (int r = 0; r < LPB; r++) {
     const int item = blockIdx.x * LPB + r;
-    if (item >= n_items) return;
+    if (item >= n_items) return; /* the same for every thread of the block */
     const int4 it = items[item];
     const double ih = 1.0 / geo[it.x].w;
-    const float4* __restrict__ src = src32 + it.y;
     float acc0 = 0.0f, acc1 = 0.0f;
-    int k = 0;
-    for (; k + 2 <= it.z; k += 2) {
-      const float4 s0 = src[k], s1 = src[k + 1];
-      const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-      const float bx = tx - s1.x, by = ty - s1.y, bz = tz - s1.z;
-      acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-      acc1 = __builtin_fmaf(s1.w, __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
-    }
-    if (k < it.z) {
-      const float4 s0 = src[k];
-      const float ax = tx - s0.x, ay = ty - s0.y, az = tz - s0.z;
-      acc0 = __builtin_fmaf(s0.w, __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+    for (int base = 0; base < it.z; base += S2M_TILE) {
+      const int n = min(S2M_TILE, it.z - base);
+      __syncthreads(); /* the previous tile's (or leaf's) readers are done */
+      for (int k = j; k < n; k += blockDim.x) {
+        const float4 q = src32[it.y + base + k];
+        sS[k] = make_float4(q.x, q.y, q.z, __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z)));
+        sW[k] = q.w;
+      }
+      __syncthreads();
+      int k = 0;
+      for (; k + 2 <= n; k += 2) {
+        const float4 s0 = sS[k], s1 = sS[k + 1];
+        const float r0 = __builtin_fmaf(ax, s0.x, __builtin_fmaf(ay, s0.y, __builtin_fmaf(az, s0.z, pp + s0.w)));
+        const float r1 = __builtin_fmaf(ax, s1.x, __builtin_fmaf(ay, s1.y, __builtin_fmaf(az, s1.z, pp + s1.w)));
+        acc0 = __builtin_fmaf(sW[k], __builtin_amdgcn_rsqf(r0), acc0);
+        acc1 = __builtin_fmaf(sW[k + 1], __builtin_amdgcn_rsqf(r1), acc1);
+      }
+      if (k < n) {
+        const float4 s0 = sS[k];
+        const float r0 = __builtin_fmaf(ax, s0.x, __builtin_fmaf(ay, s0.y, __builtin_fmaf(az, s0.z, pp + s0.w)));
+        acc0 = __builtin_fmaf(sW[k], __builtin_amdgcn_rsqf(r0), acc0);
+      }
     }
     if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
   }
 }
 
+/* L2T: lane = target of an item (<= 64 targets of one leaf); the DE surface
+ * (rows x, y, z, |p|^2) is read as wave-uniform values, the leaf's
+ * densities once per item into LDS as FP32. */
 template <int LPB>
 __global__ __launch_bounds__(64) void k_l2t_mixed(const int4* __restrict__ items, int n_items,
                                                          const double4* __restrict__ geo,
                                                          const float* __restrict__ surf_de, int ld_surf, int n_e,
                                                          const double* __restrict__ dden, int ld_e,
                                                          const double4* __restrict__ trg4, double* __restrict__ pot) {
+  __shared__ float sD[64];
   for (int r = 0; r < LPB; r++) {
     const int item = blockIdx.x * LPB + r;
-    if (item >= n_items) return;
+    if (item >= n_items) return; /* the same for the whole (one-wave) block */
     const int4 it = items[item];
     const int k = threadIdx.x;
     const double4 q = trg4[it.y + min(k, it.z - 1)];
     const double4 g = geo[it.x];
     const double ih = 1.0 / g.w;
     const float tx = (float)((q.x - g.x) * ih), ty = (float)((q.y - g.y) * ih), tz = (float)((q.z - g.z) * ih);
-    const double* __restrict__ den = dden + (size_t)it.x * ld_e;
+    const float ax = -2.0f * tx, ay = -2.0f * ty, az = -2.0f * tz;
+    const float tt = __builtin_fmaf(tx, tx, __builtin_fmaf(ty, ty, tz * tz));
+    __builtin_amdgcn_wave_barrier(); /* the previous item's reads of sD are done (one wave) */
+    if (k < n_e) sD[k] = (float)dden[(size_t)it.x * ld_e + k];
+    __builtin_amdgcn_wave_barrier();
     float acc0 = 0.0f, acc1 = 0.0f;
     int j = 0;
     for (; j + 2 <= n_e; j += 2) {
-      const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-      const float bx = tx - surf_de[j + 1], by = ty - surf_de[ld_surf + j + 1],
-                  bz = tz - surf_de[2 * ld_surf + j + 1];
-      acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
-      acc1 = __builtin_fmaf((float)den[j + 1], __builtin_amdgcn_rsqf(__builtin_fmaf(bz, bz, __builtin_fmaf(by, by, bx * bx))), acc1);
+      const float r0 = __builtin_fmaf(surf_de[j], ax, __builtin_fmaf(surf_de[ld_surf + j], ay,
+                                      __builtin_fmaf(surf_de[2 * ld_surf + j], az, tt + surf_de[3 * ld_surf + j])));
+      const float r1 = __builtin_fmaf(surf_de[j + 1], ax, __builtin_fmaf(surf_de[ld_surf + j + 1], ay,
+                                      __builtin_fmaf(surf_de[2 * ld_surf + j + 1], az, tt + surf_de[3 * ld_surf + j + 1])));
+      acc0 = __builtin_fmaf(sD[j], __builtin_amdgcn_rsqf(r0), acc0);
+      acc1 = __builtin_fmaf(sD[j + 1], __builtin_amdgcn_rsqf(r1), acc1);
     }
     if (j < n_e) {
-      const float ax = tx - surf_de[j], ay = ty - surf_de[ld_surf + j], az = tz - surf_de[2 * ld_surf + j];
-      acc0 = __builtin_fmaf((float)den[j], __builtin_amdgcn_rsqf(__builtin_fmaf(az, az, __builtin_fmaf(ay, ay, ax * ax))), acc0);
+      const float r0 = __builtin_fmaf(surf_de[j], ax, __builtin_fmaf(surf_de[ld_surf + j], ay,
+                                      __builtin_fmaf(surf_de[2 * ld_surf + j], az, tt + surf_de[3 * ld_surf + j])));
+      acc0 = __builtin_fmaf(sD[j], __builtin_amdgcn_rsqf(r0), acc0);
     }
     if (k < it.z) pot[it.y + k] += OOFP_R * ih * ((double)acc0 + (double)acc1);
   }

@@ -885,12 +885,24 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
   return KIFMM_OK;
 }
 
+/* FP32 leaf-local source records of the mixed S2M ((s - c) / h, w): they
+ * depend on the positions, fixed for the handle, and the densities, so they
+ * are written here at create time and again by kifmm_gpu_set_densities, not
+ * in every evaluation. */
+int mixed_sources(kifmm_gpu* g) {
+  if (g->n_s2m > 0)
+    hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
+                       g->src32);
+  HIP_TRY(hipGetLastError());
+  return KIFMM_OK;
+}
+
 /* Mixed path, once per evaluator: leaves per block for the mixed S2M and
- * L2T.  (The FP32 leaf-local source records k_s2m_mixed reads are written
- * by k_src_local32 at the start of every up pass, so that they follow the
- * current densities.) */
+ * L2T, and the FP32 source records. */
 int mixed_setup(kifmm_gpu* g) {
   g->mix_lpb = (g->n_s2m > 0 && (double)g->n_src / g->n_s2m < 100.0) ? 4 : 1;
+  TRY(mixed_sources(g));
+  HIP_TRY(hipStreamSynchronize(g->stream));
   return KIFMM_OK;
 }
 
@@ -1279,9 +1291,7 @@ int enqueue_far(kifmm_gpu* g, unsigned mask) {
   if ((mask & KIFMM_PHASE_UP) && g->n_s2m > 0) {
     /* sources never sit on the check surface: no r = 0 mask */
     if (g->precision == KIFMM_PREC_MIXED) {
-      /* FP32 leaf-local copies of the current sources and densities */
-      hipLaunchKernelGGL(k_src_local32, dim3(g->n_s2m), dim3(256), 0, g->stream, g->s2m_items, g->geo, g->src4,
-                         g->src32);
+      /* the FP32 leaf-local source records (mixed_sources) */
       hipLaunchKernelGGL(g->mix_lpb == 4 ? k_s2m_mixed<4> : k_s2m_mixed<1>, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb),
                          dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->n_s2m, g->geo, g->surf_uc32,
                          g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
@@ -1959,6 +1969,7 @@ int kifmm_gpu_set_densities(kifmm_gpu* g, const double* den_leaf) {
   const int blocks = (int)std::min<size_t>(4096, ((size_t)g->n_src + 255) / 256);
   hipLaunchKernelGGL(k_set_w, dim3(blocks), dim3(256), 0, g->stream, g->src4, g->den_stage, (size_t)g->n_src);
   HIP_TRY(hipGetLastError());
+  if (g->precision == KIFMM_PREC_MIXED) TRY(mixed_sources(g));
   HIP_TRY(hipStreamSynchronize(g->stream)); /* den_leaf may be released on return */
   return KIFMM_OK;
 }
