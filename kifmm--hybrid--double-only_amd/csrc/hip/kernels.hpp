@@ -771,14 +771,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   double acc[MIX_G][4];
 #pragma unroll
   for (int q = 0; q < MIX_G; q++) {
-    const int grp = min(wave + 4 * q, ngroups - 1);
-    const double4 t = sT[16 * grp + (lane & 15)];
-    const int k4 = lane >> 4;
-    a[q] = gram_a_block(t.x, t.y, t.z, k4);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
-      acc[q][e] = 0.0;
+    for (int e = 0; e < 4; e++) acc[q][e] = 0.0;
+    a[q] = v8bf{};
+    c[q] = v4f{};
+    /* operands only for the groups this wave owns (wave-uniform): a wave of
+     * a <= 64-target item has one, and the bf16 splitting of an A block is
+     * about 90 VALU instructions per lane */
+    if (wave + 4 * q < ngroups) {
+      const int grp = wave + 4 * q;
+      const double4 t = sT[16 * grp + (lane & 15)];
+      const int k4 = lane >> 4;
+      a[q] = gram_a_block(t.x, t.y, t.z, k4);
+#pragma unroll
+      for (int e = 0; e < 4; e++) c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
     }
   }
   RangeWalk walk{rng_ptr[it.x], 0, rng_ptr[it.x + 1]};
