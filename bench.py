@@ -360,7 +360,8 @@ def tuned_nonuniform(K, steps, warmup, nv):
 def large_mixed(K, steps, warmup, nv):
     """16,777,216 uniform points, seed 3, 256 points per box, at NP = 3 and 4:
     the plain FP64 path and the mixed path (one HIP graph each), Mpoints/s,
-    sampled error against the FP64 direct sum and the speedup.  The tree and
+    sampled error against the FP64 direct sum and the speedup, with the
+    default launch shapes and with each path's autotuned ones.  The tree and
     lists are built on the GPU (identical to the host build)."""
     n = 16777216
     out = {"workload": f"uniform {n} points, separate targets, seed 3, 256 pts/box"}
@@ -373,7 +374,15 @@ def large_mixed(K, steps, warmup, nv):
             err, sample = sampled_error(prob, ev, sample, nv)
             row[name] = {"mpoints_s": rate(n, ms), "ms_per_step": ms, "sampled_rel_err": err}
             del ev
+            # each path with the launch shapes and layout kifmm_gpu_autotune picks for it
+            tuner = K.GpuEvaluator(prob, precision=prec)
+            best, _ = tuner.autotune(reps=2)
+            del tuner
+            ev, ms, _ = timed(K, prob, prec, steps, warmup, graph=True, tuning=best)
+            row[name + "_tuned"] = {"mpoints_s": rate(n, ms), "ms_per_step": ms, "shapes": best}
+            del ev
         row["speedup"] = row["mixed"]["mpoints_s"] / row["fp64"]["mpoints_s"]
+        row["speedup_tuned"] = row["mixed_tuned"]["mpoints_s"] / row["fp64_tuned"]["mpoints_s"]
         out[f"np{np_}"] = row
         del prob
     return out
