@@ -62,7 +62,7 @@ enum {
   KIFMM_P2P_STATIC = 0,     /* one block per U-list item of <= 128 targets (default) */
   KIFMM_P2P_PERSISTENT = 1, /* a few blocks per CU drawing the same items from per-XCD queues */
   KIFMM_P2P_STATIC64 = 2,   /* one block per item of <= 64 targets */
-  KIFMM_P2P_SPLIT = 3       /* mixed path only: items of <= 128 targets, up to 4 groups of 16
+  KIFMM_P2P_SPLIT = 3       /* mixed path only: items of <= 128 targets, up to 2 groups of 16
                                targets per wave, each source tile split among the waves that
                                share a set of groups (pays on small leaves) */
 };
@@ -72,10 +72,13 @@ typedef struct kifmm_gpu_tuning {
   int hadamard_fb;   /* frequencies per V-list Hadamard block: 4, 8 (default) or 16 */
   int hadamard_form; /* KIFMM_HAD_*: V-list Hadamard on MFMA (default) or on the VALU */
   int overlap;       /* stream layout of the U-list's copy-out chunks: 0 all of them last,
-                        after the far field, their copies back overlapping the later
-                        chunks; 1 all of them on the handle's second stream from the
-                        start, alongside the far field (initially 1 with KIFMM_GPU_OVERLAP);
-                        2, 3 the first one or two chunks there, the rest last */
+                        after the far field, on the two streams in turn, their copies back
+                        overlapping the later chunks; 1 all of them on the handle's second
+                        stream from the start, alongside the far field (initially 1 with
+                        KIFMM_GPU_OVERLAP); 2, 3 the first one or two chunks there, the
+                        rest last; 4 all of them last, one after another on the main stream
+                        (the first copies start sooner: pays when the copy back is long
+                        against the U-list) */
   int up_form;       /* KIFMM_LEVEL_*: how each level's M2M (UE2UC stack, then UC2UE) runs */
   int down_form;     /* KIFMM_LEVEL_*: how each level's L2L + DC2DE runs */
   int reserved[2];
@@ -225,10 +228,10 @@ int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
  * evaluations with the copy back of this handle's own input, reps timed
  * runs each after one warm-up, keep the fastest of each (minimum over runs)
  * and set it.  best (may be NULL) receives the choice; ms (may be NULL, n_ms
- * entries, up to 18) receives the times in the order STATIC, PERSISTENT,
+ * entries, up to 19) receives the times in the order STATIC, PERSISTENT,
  * STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16, layout 0, layout 1, SPLIT,
  * layout 2, layout 3, up as column GEMMs, up fused, down as column GEMMs,
- * down fused; -1 for a candidate not timed. */
+ * down fused, layout 4; -1 for a candidate not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */

@@ -652,9 +652,13 @@ constexpr double MIX_CLOSE = CLOSE_FRAC;
 typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
 
 /* FP32(v) = h + m + l exactly: each remainder is exact in FP32 and the
- * last has at most 8 significant bits */
+ * last has at most 8 significant bits.  The empty asm keeps the compiler
+ * from rounding v to bf16 straight from FP64 (its f64 -> f32 round-to-odd
+ * emulation costs 9 VALU instructions per value, and an h that is not
+ * bf16(f) leaves up to 9 bits for l) */
 __device__ __forceinline__ void split3(double v, __bf16& h, __bf16& m, __bf16& l) {
-  const float f = (float)v;
+  float f = (float)v;
+  __asm__("" : "+v"(f));
   h = (__bf16)f;
   const float r1 = f - (float)h;
   m = (__bf16)r1;
@@ -732,6 +736,74 @@ __device__ __forceinline__ void mixed_tau_chunks(const v8bf (*sB)[P2P_TILE], con
   for (int q = 0; q < G; q++)
 #pragma unroll
     for (int e = 0; e < 4; e++) acc[q][e] += (double)part[q][e];
+}
+
+/* close pairs per target up to which the per-target form sums them */
+constexpr int MIX_FEW = 16;
+
+/* FP64 close-pair terms of one 16-target group (slots t0 .. t0 + 15 of sT,
+ * nt of them real, the first at target index tg0): sC[t0 + t] = sum over
+ * the listed pairs of target t of w (1 / r' - 1), in the scaled frame (g,
+ * sc).  A group's pairs are one contiguous run of cidx (targets in order).
+ * When no target of the group has more than MIX_FEW of them, four lanes
+ * serve each target (pairs s, s + 4, ... of its run on lane 16 s + t) and
+ * add their sums by two lane exchanges; otherwise the pairs are spread over
+ * the 64 lanes whatever their distribution over the 16 targets and summed
+ * per target by a segmented scan over the lanes.  Both orders are fixed:
+ * deterministic.  One wave. */
+template <bool MASK>
+__device__ __forceinline__ void mixed_close_group(const double4* __restrict__ sT, double* __restrict__ sC, int t0,
+                                                  int nt, int tg0, const int* __restrict__ cptr,
+                                                  const int* __restrict__ cidx, const double4* __restrict__ src4,
+                                                  double4 g, double sc, int lane) {
+  const int tj = lane & 15;
+  const int sj = cptr[tg0 + min(tj, nt - 1)];
+  const int ej = cptr[tg0 + min(tj, nt - 1) + 1]; /* end of target lane&15's run */
+  int most = tj < nt ? ej - sj : 0;
+#pragma unroll
+  for (int off = 1; off < 16; off <<= 1) most = max(most, __shfl_xor(most, off, 64));
+  if (__builtin_amdgcn_readfirstlane(most) <= MIX_FEW) {
+    double v = 0.0;
+    if (tj < nt) {
+      const double4 tp = sT[t0 + tj];
+      for (int k = sj + (lane >> 4); k < ej; k += 4) {
+        const double4 sp = src4[cidx[k]];
+        const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
+        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+        v += sp.w * (inv_r<MASK>(r2) - 1.0);
+      }
+    }
+    v += __shfl_xor(v, 16, 64);
+    v += __shfl_xor(v, 32, 64);
+    if (lane < 16) sC[t0 + lane] = v;
+    return;
+  }
+  if (lane < 16) sC[t0 + lane] = 0.0;
+  const int kbeg = cptr[tg0], kend = cptr[tg0 + nt];
+  for (int kb = kbeg; kb < kend; kb += 64) {
+    const int k = kb + lane;
+    const bool live = k < kend;
+    int tl = 0; /* the target of pair k: the number of runs that end at or before k */
+#pragma unroll
+    for (int j = 0; j < 16; j++) tl += __builtin_amdgcn_readlane(ej, j) <= k;
+    tl = live ? tl : 16;
+    double v = 0.0;
+    if (live) {
+      const double4 sp = src4[cidx[k]];
+      const double4 tp = sT[t0 + tl];
+      const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
+      const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
+      v = sp.w * (inv_r<MASK>(r2) - 1.0);
+    }
+#pragma unroll
+    for (int off = 1; off < 64; off <<= 1) {
+      const double u = __shfl_up(v, off, 64);
+      const int ut = __shfl_up(tl, off, 64);
+      if (lane >= off && ut == tl) v += u;
+    }
+    const int nx = __shfl_down(tl, 1, 64);
+    if (live && (lane == 63 || nx != tl)) sC[t0 + tl] += v; /* one writer per target and chunk */
+  }
 }
 
 /* Groups of 16 targets per wave: items of up to P2P_MAXT5 targets (10
@@ -814,48 +886,18 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
     else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, 0, ns >> 4, a, c, acc, lane);
     fill = next;
   }
-  /* FP64 close-pair terms of the wave's target groups.  A group's pairs are
-   * one contiguous run of cidx (targets in order), spread over the 64 lanes
-   * whatever their distribution over the 16 targets, then summed per target
-   * by a segmented scan over the lanes (fixed order: deterministic).  The
-   * per-target sums reuse the source tile's LDS once every wave is done
-   * with it (keeps the block within 40 KB: 4 blocks per CU). */
+  /* FP64 close-pair terms of the wave's target groups.  The per-target sums
+   * reuse the source tile's LDS once every wave is done with it (keeps the
+   * block within 40 KB: 4 blocks per CU). */
   static_assert(sizeof(sB) >= P2P_MAXT5 * sizeof(double), "close-pair sums in the tile");
   __syncthreads();
   double* const sC = reinterpret_cast<double*>(&sB[0][0]);
 #pragma unroll
   for (int q = 0; q < MIX_G; q++) {
     const int grp = wave + 4 * q;
-    if (grp < ngroups) {
-      const int t0 = 16 * grp, nt = min(16, it.z - t0), tg0 = it.y + t0;
-      if (lane < 16) sC[t0 + lane] = 0.0;
-      const int kbeg = cptr[tg0], kend = cptr[tg0 + nt];
-      const int ej = cptr[tg0 + min(lane & 15, nt - 1) + 1]; /* end of target lane&15's run */
-      for (int kb = kbeg; kb < kend; kb += 64) {
-        const int k = kb + lane;
-        const bool live = k < kend;
-        int tl = 0; /* the target of pair k: the number of runs that end at or before k */
-#pragma unroll
-        for (int j = 0; j < 16; j++) tl += __builtin_amdgcn_readlane(ej, j) <= k;
-        tl = live ? tl : 16;
-        double v = 0.0;
-        if (live) {
-          const double4 sp = src4[cidx[k]];
-          const double4 tp = sT[t0 + tl];
-          const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
-          const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-          v = sp.w * (inv_r<MASK>(r2) - 1.0);
-        }
-#pragma unroll
-        for (int off = 1; off < 64; off <<= 1) {
-          const double u = __shfl_up(v, off, 64);
-          const int ut = __shfl_up(tl, off, 64);
-          if (lane >= off && ut == tl) v += u;
-        }
-        const int nx = __shfl_down(tl, 1, 64);
-        if (live && (lane == 63 || nx != tl)) sC[t0 + tl] += v; /* one writer per target and chunk */
-      }
-    }
+    if (grp < ngroups)
+      mixed_close_group<MASK>(sT, sC, 16 * grp, min(16, it.z - 16 * grp), it.y + 16 * grp, cptr, cidx, src4, g, sc,
+                              lane);
   }
   const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
 #pragma unroll
@@ -882,18 +924,20 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
 /* Mixed U-list, source-split form (items of <= P2P_MAXT targets).  The
  * arithmetic is the tau kernel's (bf16 Gram MFMA distances, clamped FP32
  * rsq, listed FP64 close-pair terms); the work is divided differently.  The
- * item's 16-target groups form sets of up to MIXS_G = 4 groups (one set for
- * <= 64 targets, two for 65-128), and the 4 / sets waves of a set split
- * every staged source tile between them (contiguous chunk ranges), each
- * running all of the set's groups.  So every wave holds up to 4 groups and
- * reads each B operand once for up to 4 MFMAs, where the tau kernel gives a
- * wave of a 64-target leaf a single group.  The partial sums of a set's
- * waves are added per target in wave order through LDS, then the close-
- * pair terms.  Sums differ from the tau kernel's at rounding level (other
- * partial sums), and are bit-identical from run to run. */
-constexpr int MIXS_G = 4;
+ * item's 16-target groups form sets of up to MIXS_G = 2 groups, and the
+ * waves of a set split every staged source tile between them (contiguous
+ * chunk ranges), each running all of the set's groups: a 64-target leaf is
+ * two sets of two waves, a 16-target one a set of four.  So a wave reads
+ * each B operand once for up to 2 MFMAs, where the tau kernel gives a wave
+ * of a 64-target leaf a single group.  (Sets of 4 groups, one per 64
+ * targets, measured 40 % slower at 64 targets per leaf: 3 waves per SIMD
+ * for their registers.)  The partial sums of a set's waves are added per
+ * target in wave order through LDS, then the close-pair terms.  Sums differ
+ * from the tau kernel's at rounding level (other partial sums), and are
+ * bit-identical from run to run. */
+constexpr int MIXS_G = 2;
 template <bool MASK>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) void k_p2p_ulist_mixed_split(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_p2p_ulist_mixed_split(
     const int4* __restrict__ items, int n_items, int per_xcd, const int* __restrict__ rng_ptr,
     const int2* __restrict__ rng, const double4* __restrict__ geo, const double4* __restrict__ src4,
     const double4* __restrict__ trg4, const int* __restrict__ cptr, const int* __restrict__ cidx,
@@ -908,11 +952,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
   const double sc = (1.0 / MIX_CLOSE) / g.w; /* 1 / (tau h), a power of two */
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
-  const int ngroups = (it.z + 15) >> 4; /* 1 .. 8 */
-  const int nsets = ngroups <= MIXS_G ? 1 : 2;
-  const int wps = 4 / nsets; /* waves per set: the source split */
+  const int ngroups = (it.z + 15) >> 4;                   /* 1 .. 8 */
+  const int nsets = (ngroups + MIXS_G - 1) / MIXS_G;      /* 1 .. 4 */
+  const int wps = nsets == 1 ? 4 : nsets == 2 ? 2 : 1;    /* waves per set: the source split */
   const int set = wave / wps, part = wave % wps;
-  const int g0 = MIXS_G * set, gn = min(MIXS_G, ngroups - g0); /* gn >= 1 for every set */
+  const int g0 = MIXS_G * set, gn = min(MIXS_G, ngroups - g0); /* gn < 1: a wave with no set (3 sets) */
   for (int k = tid; k < ngroups * 16; k += 256) {
     const double4 q = trg4[it.y + min(k, it.z - 1)];
     const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
@@ -958,10 +1002,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
     const int npairs = ns >> 5; /* chunk pairs of the tile, split among the set's waves */
     const int lo = 2 * (npairs * part / wps), hi = 2 * (npairs * (part + 1) / wps);
-    if (gn == 4) mixed_tau_chunks<4>(sB, sW, lo, hi, a, c, acc, lane);
-    else if (gn == 3) mixed_tau_chunks<3>(sB, sW, lo, hi, a, c, acc, lane);
-    else if (gn == 2) mixed_tau_chunks<2>(sB, sW, lo, hi, a, c, acc, lane);
-    else mixed_tau_chunks<1>(sB, sW, lo, hi, a, c, acc, lane);
+    if (gn == 2) mixed_tau_chunks<2>(sB, sW, lo, hi, a, c, acc, lane);
+    else if (gn == 1) mixed_tau_chunks<1>(sB, sW, lo, hi, a, c, acc, lane);
     fill = next;
   }
   /* the tile's LDS now holds the per-wave partial sums red[wave][target] and
@@ -986,36 +1028,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) voi
     }
   }
   /* FP64 close-pair terms, group grp by wave grp % 4 (as in the tau kernel) */
-  for (int grp = wave; grp < ngroups; grp += 4) {
-    const int t0 = 16 * grp, nt = min(16, it.z - t0), tg0 = it.y + t0;
-    if (lane < 16) sC[t0 + lane] = 0.0;
-    const int kbeg = cptr[tg0], kend = cptr[tg0 + nt];
-    const int ej = cptr[tg0 + min(lane & 15, nt - 1) + 1];
-    for (int kb = kbeg; kb < kend; kb += 64) {
-      const int k = kb + lane;
-      const bool live = k < kend;
-      int tl = 0;
-#pragma unroll
-      for (int j = 0; j < 16; j++) tl += __builtin_amdgcn_readlane(ej, j) <= k;
-      tl = live ? tl : 16;
-      double v = 0.0;
-      if (live) {
-        const double4 sp = src4[cidx[k]];
-        const double4 tp = sT[t0 + tl];
-        const double dx = tp.x - (sp.x - g.x) * sc, dy = tp.y - (sp.y - g.y) * sc, dz = tp.z - (sp.z - g.z) * sc;
-        const double r2 = __builtin_fma(dz, dz, __builtin_fma(dy, dy, dx * dx));
-        v = sp.w * (inv_r<MASK>(r2) - 1.0);
-      }
-#pragma unroll
-      for (int off = 1; off < 64; off <<= 1) {
-        const double u = __shfl_up(v, off, 64);
-        const int ut = __shfl_up(tl, off, 64);
-        if (lane >= off && ut == tl) v += u;
-      }
-      const int nx = __shfl_down(tl, 1, 64);
-      if (live && (lane == 63 || nx != tl)) sC[t0 + tl] += v;
-    }
-  }
+  for (int grp = wave; grp < ngroups; grp += 4)
+    mixed_close_group<MASK>(sT, sC, 16 * grp, min(16, it.z - 16 * grp), it.y + 16 * grp, cptr, cidx, src4, g, sc,
+                            lane);
   __syncthreads();
   const double scale = OOFP_R * sc;
   for (int t = tid; t < it.z; t += 256) {

@@ -1203,8 +1203,9 @@ int launch_u(kifmm_gpu* g, hipStream_t st, int c0, int c1, double* out, int accu
 
 /* Copy-out chunks whose U-list runs on the second stream from the start of
  * the evaluation (alongside the far field) rather than after it: none in
- * the default layout (tuning.overlap 0), all of them in the overlap layout
- * (1), the first one or two (2, 3) in the split layouts. */
+ * the default layout (tuning.overlap 0) and the sequential one (4), all of
+ * them in the overlap layout (1), the first one or two (2, 3) in the split
+ * layouts. */
 int early_chunks(const kifmm_gpu* g, unsigned mask) {
   if (!(mask & KIFMM_PHASE_U)) return 0;
   const int nch = (int)g->cuts.size() - 1;
@@ -1378,13 +1379,18 @@ int enqueue_near(kifmm_gpu* g, unsigned mask, double* dst) {
   }
   if (e < nch) { /* the other chunks after the far field */
     const int late = dst ? nch - e : 1; /* without a copy: one launch */
-    if (late > 1) {
+    /* on the two streams in turn (one chunk's last blocks run beside the
+     * next chunk's first), or in layout 4 one after another on the main
+     * stream: the first copies then start sooner, which pays when the copy
+     * back is long against the U-list */
+    const bool pairs = late > 1 && g->overlap != 4;
+    if (pairs) {
       HIP_TRY(hipEventRecord(g->far_ev, g->stream));
       HIP_TRY(hipStreamWaitEvent(g->side, g->far_ev, 0));
     }
     for (int k = 0; k < late; k++) {
       const int c = e + k;
-      hipStream_t st = (k & 1) ? g->side : g->stream;
+      hipStream_t st = (pairs && (k & 1)) ? g->side : g->stream;
       TRY(launch_u(g, st, c, dst ? c + 1 : nch, g->pot, 1));
       if (dst) {
         HIP_TRY(hipEventRecord(g->ch_ev[c], st));
@@ -1393,7 +1399,7 @@ int enqueue_near(kifmm_gpu* g, unsigned mask, double* dst) {
         HIP_TRY(hipEventRecord(g->cd_ev[c], g->cstream));
       }
     }
-    if (late > 1) {
+    if (pairs) {
       HIP_TRY(hipEventRecord(g->sdone_ev, g->side));
       HIP_TRY(hipStreamWaitEvent(g->stream, g->sdone_ev, 0));
     }
@@ -1801,10 +1807,10 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
   if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_SPLIT ||
       (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
       (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA) ||
-      t->overlap < 0 || t->overlap > 3 || t->up_form < KIFMM_LEVEL_AUTO || t->up_form > KIFMM_LEVEL_FUSED ||
+      t->overlap < 0 || t->overlap > 4 || t->up_form < KIFMM_LEVEL_AUTO || t->up_form > KIFMM_LEVEL_FUSED ||
       t->down_form < KIFMM_LEVEL_AUTO || t->down_form > KIFMM_LEVEL_FUSED) {
     set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16, "
-              "hadamard_form a KIFMM_HAD_* value, overlap 0 to 3 and up_form / down_form KIFMM_LEVEL_* "
+              "hadamard_form a KIFMM_HAD_* value, overlap 0 to 4 and up_form / down_form KIFMM_LEVEL_* "
               "values");
     return KIFMM_EINVAL;
   }
@@ -1856,8 +1862,8 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[18];
-  std::fill(t, t + 18, -1.0);
+  double t[19];
+  std::fill(t, t + 19, -1.0);
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
   /* the fastest of reps timed runs after one warm-up: the U-list, the
@@ -1924,10 +1930,10 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
    * would otherwise capture here) */
   const int use_graph = g->use_graph;
   g->use_graph = 0;
-  /* slots of the layouts 0 .. 3 in t */
-  const int lslot[4] = {9, 10, 12, 13};
+  /* slots of the layouts 0 .. 4 in t */
+  const int lslot[5] = {9, 10, 12, 13, 18};
   int best_l = 0, rc = KIFMM_OK;
-  for (int l = 0; l < 4 && rc == KIFMM_OK; l++) {
+  for (int l = 0; l < 5 && rc == KIFMM_OK; l++) {
     c.overlap = l;
     rc = kifmm_gpu_set_tuning(g, &c);
     kifmm_phase_times pt{};
@@ -1943,7 +1949,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 18); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 19); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

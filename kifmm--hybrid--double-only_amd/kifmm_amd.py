@@ -468,11 +468,11 @@ class GpuEvaluator:
         Hadamard on the VALU with 4, 8, 16 frequencies per block and on MFMA
         with 4, 8, 16, then whole evaluations with the copy back in stream
         layouts 0 and 1, then SPLIT (mixed path), then layouts 2 and 3, then
-        up as column GEMMs and fused, down as column GEMMs and fused; -1
-        where not timed."""
+        up as column GEMMs and fused, down as column GEMMs and fused, then
+        layout 4; -1 where not timed."""
         t = GpuTuning()
-        ms = np.zeros(18)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 18))
+        ms = np.zeros(19)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 19))
         return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
                 "overlap": t.overlap, "up_form": t.up_form, "down_form": t.down_form}, ms
 

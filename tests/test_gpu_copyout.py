@@ -35,7 +35,7 @@ def test_copy_out_is_the_device_result(prec):
     assert np.array_equal(own, dev)
     got, _ = ev.evaluate()  # numpy copy of the pinned buffer
     assert np.array_equal(got, dev)
-    for ov in (1, 2, 3):  # U-list chunks on the second stream from the start: all, first, first two
+    for ov in (1, 2, 3, 4):  # on the second stream from the start: all, first, first two; 4: one stream
         ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, ov)
         got, t = ev.evaluate()
         assert np.array_equal(got, dev), ov
@@ -47,7 +47,7 @@ def test_copy_out_is_the_device_result(prec):
     forms = (K.P2P_PERSISTENT, K.P2P_STATIC64) if prec == K.PREC_FP64 else (K.P2P_SPLIT,)
     split = None
     for f in forms:
-        for ov in (0, 1, 2):
+        for ov in (0, 1, 2, 4):
             ev.set_tuning(f, 8, K.HAD_MFMA, ov)
             got, _ = ev.evaluate()
             if f == K.P2P_SPLIT:  # its own sums (rounding-level differences), the same in every layout
@@ -56,7 +56,7 @@ def test_copy_out_is_the_device_result(prec):
             else:
                 assert np.array_equal(got, dev), (f, ov)
     gr = K.GpuEvaluator(prob, precision=prec, graph=True)
-    for k in range(3):
+    for k in range(5):
         gr.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, k)  # a recapture per layout
         got, _ = gr.evaluate()
         assert np.array_equal(got, dev), ("graph replay", k)
