@@ -810,6 +810,61 @@ __device__ __forceinline__ void mixed_close_group(const double4* __restrict__ sT
  * groups) give a wave at most 3. */
 constexpr int MIX_G = 3;
 static_assert(4 * 16 * MIX_G >= P2P_MAXT5, "mixed U-list groups");
+
+/* mixed_tau_chunks over the first nch chunks of the tile for the wave's
+ * groups q with bit q of M set (a compile-time set, so that the operand
+ * arrays stay in registers) */
+template <int M>
+__device__ __forceinline__ void mixed_tau_masked(const v8bf (*sB)[P2P_TILE], const float* __restrict__ sW, int nch,
+                                                 const v8bf (&a)[MIX_G], const v4f (&c)[MIX_G],
+                                                 double (&acc)[MIX_G][4], int lane) {
+  const int j = lane & 15;
+  const v8bf* __restrict__ col = sB[lane >> 4];
+  float part[MIX_G][4];
+#pragma unroll
+  for (int q = 0; q < MIX_G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) part[q][e] = 0.0f;
+  for (int ch = 0; ch < nch; ch += 2) {
+#pragma unroll
+    for (int u = 0; u < 2; u++) {
+      const int s = (ch + u) * 16 + j;
+      const v8bf b = col[s];
+      const float w = sW[s];
+#pragma unroll
+      for (int q = 0; q < MIX_G; q++) {
+        if (((M >> q) & 1) != 0) {
+          const v4f d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[q], b, c[q], 0, 0, 0);
+#pragma unroll
+          for (int e = 0; e < 4; e++)
+            part[q][e] = __builtin_fmaf(
+                w, __builtin_amdgcn_fmed3f(__builtin_amdgcn_rsqf(__builtin_fabsf(d[e])), 0.0f, 1.0f), part[q][e]);
+        }
+      }
+    }
+  }
+  /* every group, the others adding their zero partial sums: the same code
+   * in every instantiation (per-group adds under the mask let the compiler
+   * merge them across the switch below into stores at a variable index,
+   * which puts acc in scratch memory) */
+#pragma unroll
+  for (int q = 0; q < MIX_G; q++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) acc[q][e] += (double)part[q][e];
+}
+__device__ __forceinline__ void mixed_tau_range(const v8bf (*sB)[P2P_TILE], const float* __restrict__ sW, int nch,
+                                                int mask, const v8bf (&a)[MIX_G], const v4f (&c)[MIX_G],
+                                                double (&acc)[MIX_G][4], int lane) {
+  switch (mask) {
+    case 1: mixed_tau_masked<1>(sB, sW, nch, a, c, acc, lane); break;
+    case 2: mixed_tau_masked<2>(sB, sW, nch, a, c, acc, lane); break;
+    case 3: mixed_tau_masked<3>(sB, sW, nch, a, c, acc, lane); break;
+    case 4: mixed_tau_masked<4>(sB, sW, nch, a, c, acc, lane); break;
+    case 6: mixed_tau_masked<6>(sB, sW, nch, a, c, acc, lane); break;
+    case 7: mixed_tau_masked<7>(sB, sW, nch, a, c, acc, lane); break;
+    default: break; /* 0: the wave has no group in this segment */
+  }
+}
 template <bool MASK>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) void k_p2p_ulist_mixed_tau(const int4* __restrict__ items, int n_items, int per_xcd,
                                                              const int* __restrict__ rng_ptr,
@@ -826,14 +881,25 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   const int item = (int)(blockIdx.x & 7) * per_xcd + (int)(blockIdx.x >> 3);
   if (item >= n_items) return; /* whole block, before any barrier */
   const int4 it = items[item];
-  const double4 g = geo[it.w];
+  /* a leaf (nB = 0) or two sibling leaves A = it.w and B = it.w + 1 whose
+   * targets follow each other: groups 0 .. gA - 1 are A's, the rest B's,
+   * and the frame is centred between the two leaves */
+  const int nA = it.z & 0xffff, nB = it.z >> 16;
+  const int gA = (nA + 15) >> 4, ngroups = gA + ((nB + 15) >> 4);
+  double4 g = geo[it.w];
+  if (nB > 0) {
+    const double4 g2 = geo[it.w + 1];
+    g.x = 0.5 * (g.x + g2.x);
+    g.y = 0.5 * (g.y + g2.y);
+    g.z = 0.5 * (g.z + g2.z);
+  }
   const double sc = (1.0 / MIX_CLOSE) / g.w; /* 1 / (tau h), a power of two */
   const int tid = threadIdx.x, lane = tid & 63;
   const int wave = (__builtin_amdgcn_readfirstlane(tid >> 6) + item) & 3; /* rotated as in p2p_item */
-  const int ngroups = (it.z + 15) >> 4;
-  /* scaled targets; slots past the count repeat the last target */
+  /* scaled targets; slots past a leaf's count repeat its last target */
   for (int k = tid; k < ngroups * 16; k += 256) {
-    const double4 q = trg4[it.y + min(k, it.z - 1)];
+    const int tix = k < 16 * gA ? min(k, nA - 1) : nA + min(k - 16 * gA, nB - 1);
+    const double4 q = trg4[it.y + tix];
     const double x = (q.x - g.x) * sc, y = (q.y - g.y) * sc, z = (q.z - g.z) * sc;
     sT[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
   }
@@ -859,9 +925,28 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       for (int e = 0; e < 4; e++) c[q][e] = (float)sT[16 * grp + 4 * k4 + e].w;
     }
   }
+  /* the wave's groups wave + 4q: q < qa are A's, qa <= q < qn B's */
+  const int qn = wave < ngroups ? (ngroups - wave + 3) >> 2 : 0;
+  const int qa = wave < gA ? (gA - wave + 3) >> 2 : 0;
+  /* source segments, each its own run of sets in rng_ptr and its own tiles:
+   * a leaf's U-list; or for a pair the boxes in both U-lists (every group),
+   * those only in A's (A's groups) and those only in B's (B's groups) */
+  const int nseg = nB > 0 ? 3 : 1;
+  int seg = 0;
   RangeWalk walk{rng_ptr[it.x], 0, rng_ptr[it.x + 1]};
   double4 pre[P2P_PER];
-  int fill = walk_tile(walk, rng, src4, tid, pre);
+  auto next_tile = [&](int& tseg) {
+    int f = walk_tile(walk, rng, src4, tid, pre);
+    while (f == 0 && seg + 1 < nseg) {
+      seg++;
+      walk = RangeWalk{rng_ptr[it.x + seg], 0, rng_ptr[it.x + seg + 1]};
+      f = walk_tile(walk, rng, src4, tid, pre);
+    }
+    tseg = seg;
+    return f;
+  };
+  int tseg = 0;
+  int fill = next_tile(tseg);
   while (fill > 0) {
     __syncthreads(); /* the previous tile's readers are done */
 #pragma unroll
@@ -880,11 +965,12 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
       sW[k] = 0.0f;
     }
     __syncthreads();
-    const int next = walk_tile(walk, rng, src4, tid, pre); /* loads in flight during the tile */
-    if (wave + 8 < ngroups) mixed_tau_chunks<3>(sB, sW, 0, ns >> 4, a, c, acc, lane);
-    else if (wave + 4 < ngroups) mixed_tau_chunks<2>(sB, sW, 0, ns >> 4, a, c, acc, lane);
-    else if (wave < ngroups) mixed_tau_chunks<1>(sB, sW, 0, ns >> 4, a, c, acc, lane);
+    int nseg_next = 0;
+    const int next = next_tile(nseg_next); /* loads in flight during the tile */
+    const int q0 = tseg == 2 ? qa : 0, q1 = tseg == 1 ? qa : qn; /* wave-uniform */
+    mixed_tau_range(sB, sW, ns >> 4, ((1 << q1) - 1) & ~((1 << q0) - 1), a, c, acc, lane);
     fill = next;
+    tseg = nseg_next;
   }
   /* FP64 close-pair terms of the wave's target groups.  The per-target sums
    * reuse the source tile's LDS once every wave is done with it (keeps the
@@ -892,11 +978,14 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   static_assert(sizeof(sB) >= P2P_MAXT5 * sizeof(double), "close-pair sums in the tile");
   __syncthreads();
   double* const sC = reinterpret_cast<double*>(&sB[0][0]);
+  /* group grp's first target (offset from it.y) and target count */
+  auto grp_first = [&](int grp) { return grp < gA ? 16 * grp : nA + 16 * (grp - gA); };
+  auto grp_count = [&](int grp) { return grp < gA ? min(16, nA - 16 * grp) : min(16, nB - 16 * (grp - gA)); };
 #pragma unroll
   for (int q = 0; q < MIX_G; q++) {
     const int grp = wave + 4 * q;
     if (grp < ngroups)
-      mixed_close_group<MASK>(sT, sC, 16 * grp, min(16, it.z - 16 * grp), it.y + 16 * grp, cptr, cidx, src4, g, sc,
+      mixed_close_group<MASK>(sT, sC, 16 * grp, grp_count(grp), it.y + grp_first(grp), cptr, cidx, src4, g, sc,
                               lane);
   }
   const double scale = OOFP_R * sc; /* 1/(4 pi) sum w / r = 1/(4 pi tau h) sum w / r' */
@@ -904,6 +993,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
   for (int q = 0; q < MIX_G; q++) {
     const int grp = wave + 4 * q;
     if (grp < ngroups) {
+      const int first = grp_first(grp), cnt = grp_count(grp);
 #pragma unroll
       for (int e = 0; e < 4; e++) {
         double v = acc[q][e];
@@ -911,10 +1001,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 8))) voi
         v += __shfl_xor(v, 4, 64);
         v += __shfl_xor(v, 2, 64);
         v += __shfl_xor(v, 1, 64);
-        const int t = 16 * grp + 4 * (lane >> 4) + e;
-        if ((lane & 15) == 0 && t < it.z) {
-          const double u = scale * (v + sC[t]);
-          pot[it.y + t] = accum ? u + pot[it.y + t] : u;
+        const int r = 4 * (lane >> 4) + e; /* the target's row in the group */
+        if ((lane & 15) == 0 && r < cnt) {
+          const double u = scale * (v + sC[16 * grp + r]);
+          double* const o = pot + it.y + first + r;
+          *o = accum ? u + *o : u;
         }
       }
     }

@@ -96,7 +96,8 @@ struct kifmm_gpu {
   /* work lists */
   int4* p2p_items = nullptr; int n_p2p = 0;
   int4* p2p_items64 = nullptr; int n_p2p64 = 0;
-  int4* p2p_items5 = nullptr; int n_p2p5 = 0; /* <= P2P_MAXT5 targets: the FP64 Gram-form and mixed tau kernels */
+  int4* p2p_items5 = nullptr; int n_p2p5 = 0; /* <= P2P_MAXT5 targets: the FP64 Gram-form kernels */
+  int4* p2p_itemsP = nullptr; int n_p2pP = 0; /* sibling pairs and single leaves: the mixed tau kernel */
   int* rng_ptr = nullptr; int2* rng = nullptr;
   int* p2p_counters = nullptr; /* 8, one per XCD: persistent U-list */
   int persist_grid[2] = {0, 0}; /* blocks of the persistent U-list form: difference, Gram */
@@ -143,7 +144,7 @@ struct kifmm_gpu {
    * the first item of each chunk per item list, the copy stream (default
    * priority) and the events that order chunks, copies and joins */
   std::vector<int> cuts;
-  std::vector<int> chunk_beg[3];
+  std::vector<int> chunk_beg[4];
   hipStream_t cstream = nullptr;
   std::vector<hipEvent_t> ch_ev, cd_ev;
   hipEvent_t far_ev = nullptr, sdone_ev = nullptr, cdone_ev = nullptr;
@@ -297,12 +298,12 @@ struct HostLevel {
 struct Plan {
   int np = 0, n_e = 0, n_uc = 0, ld_e = 0, ld_uc = 0, lda_e = 0, lda_uc = 0, lda_l2l = 0, ld_surf = 0;
   int N = 0, H = 0, F = 0, nblk = 0, max_depth = 0;
-  std::vector<int4> p2p_items, p2p_items64, p2p_items5;
+  std::vector<int4> p2p_items, p2p_items64, p2p_items5, p2p_itemsP;
   /* copy-out chunks: target cut points (leaf starts, 0 .. n_trg) and, per
-   * item list (<= P2P_MAXT, <= P2P_MAXT / 2, <= P2P_MAXT5), the first item
-   * of each chunk */
+   * item list (<= P2P_MAXT, <= P2P_MAXT / 2, <= P2P_MAXT5, pairs), the
+   * first item of each chunk */
   std::vector<int> cuts;
-  std::vector<int> chunk_beg[3];
+  std::vector<int> chunk_beg[4];
   std::vector<int> rng_ptr;
   std::vector<int2> rng;
   std::vector<int4> s2m_items;
@@ -324,6 +325,85 @@ bool offsets_ok(const std::vector<int>& off, long len_read, long buf_len) {
   for (int o : off)
     if (o < 0 || (long)o + len_read > buf_len) return false;
   return true;
+}
+
+/* Targets of a U-list work item: it.z holds n_A | n_B << 16, with n_B = 0
+ * but for the sibling pairs of the pair list. */
+inline int item_targets(const int4& it) { return (it.z & 0xffff) + (it.z >> 16); }
+
+/* The mixed tau kernel's item list.  Two sibling leaves A and B = A + 1
+ * (octants 2m and 2m + 1 of one parent, so adjacent) at one depth, whose
+ * targets follow each other and fill at most P2P_MAXT5 / 16 groups of 16
+ * (each leaf's own groups), are one item: three source sets follow each
+ * other in rng_ptr (the boxes in both U-lists, in A's order; those only in
+ * A's; those only in B's) and the item is (first set, A's first target,
+ * n_A | n_B << 16, A).  Every other leaf keeps its items of the P2P_MAXT5
+ * list (its own set, n_B = 0).  A pair stages each shared source once for
+ * the targets of both leaves: 36 boxes of sources instead of 2 x 27 for
+ * same-size neighbours. */
+int pair_items(const kifmm_tree_view* t, Plan& P) {
+  const int nn = t->n_nodes;
+  auto trg_leaf = [&](int i) { return i >= 0 && i < nn && t->child[i] == -1 && t->trg_num[i] > 0; };
+  std::vector<char> mark(nn, 0);
+  std::vector<int> sh, oa, ob;
+  auto add_set = [&](const std::vector<int>& boxes) {
+    for (int s : boxes) {
+      const int b = t->src_beg[s], n = t->src_num[s];
+      if (n <= 0) continue;
+      if ((int)P.rng.size() > P.rng_ptr.back() && P.rng.back().x + P.rng.back().y == b) P.rng.back().y += n;
+      else P.rng.push_back(make_int2(b, n));
+    }
+    P.rng_ptr.push_back((int)P.rng.size());
+  };
+  const std::vector<int4>& L5 = P.p2p_items5; /* in leaf order */
+  size_t k5 = 0;
+  P.p2p_itemsP.clear();
+  for (int i = 0; i < nn; i++) {
+    if (!trg_leaf(i)) continue;
+    const int j = i + 1, nA = t->trg_num[i];
+    const bool pair = i >= 1 && (i - 1) % 2 == 0 && trg_leaf(j) && (i - 1) / 8 == (j - 1) / 8 &&
+                      t->depth[i] == t->depth[j] && t->trg_beg[j] == t->trg_beg[i] + nA &&
+                      (nA + 15) / 16 + (t->trg_num[j] + 15) / 16 <= P2P_MAXT5 / 16;
+    while (k5 < L5.size() && L5[k5].w < i) k5++;
+    if (!pair) {
+      while (k5 < L5.size() && L5[k5].w == i) P.p2p_itemsP.push_back(L5[k5++]);
+      continue;
+    }
+    sh.clear();
+    oa.clear();
+    ob.clear();
+    for (int k = t->u_ptr[j]; k < t->u_ptr[j + 1]; k++) mark[t->u_idx[k]] = 1;
+    for (int k = t->u_ptr[i]; k < t->u_ptr[i + 1]; k++) {
+      const int s = t->u_idx[k];
+      if (mark[s]) {
+        sh.push_back(s);
+        mark[s] = 2;
+      } else {
+        oa.push_back(s);
+      }
+    }
+    for (int k = t->u_ptr[j]; k < t->u_ptr[j + 1]; k++) {
+      const int s = t->u_idx[k];
+      if (mark[s] == 1) ob.push_back(s);
+      mark[s] = 0;
+    }
+    const int set = (int)P.rng_ptr.size() - 1;
+    add_set(sh);
+    add_set(oa);
+    add_set(ob);
+    P.p2p_itemsP.push_back(make_int4(set, t->trg_beg[i], nA | (t->trg_num[j] << 16), i));
+    i = j; /* B is done */
+  }
+  for (const int4& it : P.p2p_itemsP) {
+    const int nA = it.z & 0xffff, nB = it.z >> 16, sets = nB ? 3 : 1;
+    if (it.x < 0 || it.x + sets >= (int)P.rng_ptr.size() || nA < 1 ||
+        (nA + 15) / 16 + (nB + 15) / 16 > P2P_MAXT5 / 16 || it.y < 0 || (long)it.y + nA + nB > t->n_trg ||
+        it.w < 0 || it.w + (nB ? 1 : 0) >= nn) {
+      set_error("internal: U-list pair item out of range");
+      return KIFMM_EINVAL;
+    }
+  }
+  return KIFMM_OK;
 }
 
 int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
@@ -383,6 +463,7 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       if (it.x < 0 || it.x + 1 >= (int)P.rng_ptr.size() || it.z < 1 || it.z > cap || it.y < 0 ||
           (long)it.y + it.z > t->n_trg || it.w < 0 || it.w >= nn)
         return fail("U-list work item out of range");
+  TRY(pair_items(t, P));
   for (const int2& r : P.rng)
     if (r.x < 0 || r.y < 1 || (long)r.x + r.y > t->n_src) return fail("U-list source range out of range");
   /* Copy-out chunks.  The U-list runs last, chunk by chunk, and each
@@ -393,13 +474,13 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
    * chunks shrink towards the end so that the copy left after the last
    * chunk is short.  Small inputs are one chunk. */
   {
-    const std::vector<int4>* lists[3] = {&P.p2p_items, &P.p2p_items64, &P.p2p_items5};
+    const std::vector<int4>* lists[4] = {&P.p2p_items, &P.p2p_items64, &P.p2p_items5, &P.p2p_itemsP};
     bool tiled = t->n_trg > 0;
     for (const std::vector<int4>* L : lists) {
       long next = 0;
       for (const int4& it : *L) {
         if (it.y != next) tiled = false;
-        next = (long)it.y + it.z;
+        next = (long)it.y + item_targets(it);
       }
       if (next != t->n_trg) tiled = false;
     }
@@ -408,9 +489,10 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       static const double frac[] = {0.30, 0.55, 0.75, 0.88, 0.95};
       for (double f : frac) {
         const long want = (long)(f * t->n_trg);
-        /* the first leaf start at or after want */
+        /* the first leaf start at or after want that starts an item of
+         * the pair list (then every list has an item boundary there) */
         int cut = -1;
-        for (const int4& it : P.p2p_items5)
+        for (const int4& it : P.p2p_itemsP)
           if (it.y >= want && it.y == t->trg_beg[it.w]) {
             cut = it.y;
             break;
@@ -419,7 +501,7 @@ int make_plan(const kifmm_tree_view* t, const kifmm_operators* ops, Plan& P) {
       }
     }
     P.cuts.push_back(t->n_trg);
-    for (int l = 0; l < 3; l++) {
+    for (int l = 0; l < 4; l++) {
       const std::vector<int4>& L = *lists[l];
       P.chunk_beg[l].clear();
       size_t k = 0;
@@ -822,9 +904,11 @@ int build(kifmm_gpu* g, const kifmm_tree_view* t, const kifmm_operators* ops, co
   TRY(dev_upload(g, &g->p2p_items64, P.p2p_items64.data(), P.p2p_items64.size()));
   g->n_p2p5 = (int)P.p2p_items5.size();
   TRY(dev_upload(g, &g->p2p_items5, P.p2p_items5.data(), P.p2p_items5.size()));
+  g->n_p2pP = (int)P.p2p_itemsP.size();
+  TRY(dev_upload(g, &g->p2p_itemsP, P.p2p_itemsP.data(), P.p2p_itemsP.size()));
   TRY(dev_zero(g, &g->p2p_counters, 8 * std::max<size_t>(P.cuts.size(), 1)));
   g->cuts = P.cuts;
-  for (int l = 0; l < 3; l++) g->chunk_beg[l] = P.chunk_beg[l];
+  for (int l = 0; l < 4; l++) g->chunk_beg[l] = P.chunk_beg[l];
   TRY(dev_zero(g, &g->potU, (size_t)t->n_trg));
   TRY(dev_zero(g, &g->den_stage, (size_t)t->n_src));
   {
@@ -1135,12 +1219,13 @@ int vlist_phase(kifmm_gpu* g, hipEvent_t e1, hipEvent_t e2, hipEvent_t e3) {
   return vlist_phase<NP, double2>(g, g->X, g->Y, g->T, e1, e2, e3);
 }
 
-/* The item list the U-list launch uses: the Gram-form and mixed tau kernels
- * take items of <= P2P_MAXT5 targets (list 2), the STATIC64 shape items of
- * <= 64 (list 1), the difference form and the mixed in-loop test items of
- * <= P2P_MAXT (list 0). */
+/* The item list the U-list launch uses: the mixed tau kernel takes sibling
+ * pairs and single leaves (list 3), the FP64 Gram-form kernels items of
+ * <= P2P_MAXT5 targets (list 2), the STATIC64 shape items of <= 64 (list
+ * 1), the difference form, the mixed source-split form and the mixed
+ * in-loop test items of <= P2P_MAXT (list 0). */
 int u_list_index(const kifmm_gpu* g) {
-  if (g->precision == KIFMM_PREC_MIXED) return g->close_listed && g->p2p_form != KIFMM_P2P_SPLIT ? 2 : 0;
+  if (g->precision == KIFMM_PREC_MIXED) return g->close_listed && g->p2p_form != KIFMM_P2P_SPLIT ? 3 : 0;
   if (g->p2p_form == KIFMM_P2P_STATIC64) return 1;
   return g->trg_gram ? 2 : 0;
 }
@@ -1151,7 +1236,7 @@ int u_list_index(const kifmm_gpu* g) {
 template <bool MASK>
 int launch_u(kifmm_gpu* g, hipStream_t st, int c0, int c1, double* out, int accum) {
   const int l = u_list_index(g);
-  const int4* list = l == 0 ? g->p2p_items : l == 1 ? g->p2p_items64 : g->p2p_items5;
+  const int4* list = l == 0 ? g->p2p_items : l == 1 ? g->p2p_items64 : l == 2 ? g->p2p_items5 : g->p2p_itemsP;
   const int beg = g->chunk_beg[l][c0], end = g->chunk_beg[l][c1];
   const int chunk = c0;
   const int n = end - beg, per = (n + 7) / 8;
