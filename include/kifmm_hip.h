@@ -143,7 +143,8 @@ typedef struct kifmm_work_counts {
                           listed at create time; -1 when not listed (KIFMM_GPU_NO_CLOSE_LIST
                           or too many) */
   double copy_chunks;  /* U-list chunks whose copies back overlap the later chunks */
-  double reserved[3];
+  double pair_items;   /* sibling-leaf pairs among the mixed U-list's work items */
+  double reserved[2];
 } kifmm_work_counts;
 
 int kifmm_gpu_create(const kifmm_tree_view* tree, const kifmm_operators* ops,

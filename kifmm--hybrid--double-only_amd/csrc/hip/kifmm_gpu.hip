@@ -392,6 +392,7 @@ int pair_items(const kifmm_tree_view* t, Plan& P) {
     add_set(oa);
     add_set(ob);
     P.p2p_itemsP.push_back(make_int4(set, t->trg_beg[i], nA | (t->trg_num[j] << 16), i));
+    P.wc.pair_items += 1.0;
     i = j; /* B is done */
   }
   for (const int4& it : P.p2p_itemsP) {

@@ -93,7 +93,8 @@ class PhaseTimes(C.Structure):
 class WorkCounts(C.Structure):
     _fields_ = [(n, C.c_double) for n in (
         "u_pairs", "s2m_pairs", "l2t_pairs", "w_pairs", "x_pairs", "v_entries", "v_cmac", "v_cmac_ref",
-        "gemm_flops", "fft_bytes", "hadamard_bytes", "close_pairs", "copy_chunks")] + [("reserved", C.c_double * 3)]
+        "gemm_flops", "fft_bytes", "hadamard_bytes", "close_pairs", "copy_chunks", "pair_items")] + \
+        [("reserved", C.c_double * 2)]
 
     def as_dict(self):
         return {n: getattr(self, n) for n, _ in self._fields_ if n != "reserved"}

@@ -96,6 +96,18 @@ def test_work_counts_match_reference_estimators():
     assert w["v_entries"] == t["v_ptr"][-1] == 640584
 
 
+def test_plan_pairs_sibling_leaves():
+    """The mixed U-list takes two sibling leaves per work item when their
+    targets fill at most 10 groups of 16 (each leaf its own groups): every
+    leaf of a uniform 16-point-per-leaf tree is in a pair, none of a
+    128-point-per-leaf one."""
+    for q, want in ((64, 0.5), (256, 0.0)):
+        p = K.Problem.create(65536, "uniform", q, np_=4, seed=0)
+        t = p.tree()
+        leaves = int(np.sum((t["child"] == -1) & (t["trg_num"] > 0)))
+        assert K.gpu_plan(p)["pair_items"] == want * leaves, q
+
+
 @pytest.mark.skipif(K.gpu_available(), reason="checks the no-device error path")
 def test_create_without_device_returns_error():
     p = K.Problem.create(4096, "uniform", 64, np_=4, seed=1)
