@@ -1784,15 +1784,109 @@ __global__ __launch_bounds__(256) void k_m2m(int ne, int lde, int lduc, const do
   }
 }
 
+/* Column GEMM for a short operator over many columns: M <= 64 rows, K <=
+ * GEMM_SK and a multiple of 8 (the evaluator's leaf UC2UE at NP <= 4, 56 x
+ * 152 over one column per leaf).  out[:, j] = (accumulate ? out[:, j] : 0)
+ * + 2^sexp[j] A in[:, j], as k_gemm_cols.  A block of 8 waves stages the
+ * whole operator in LDS once ([k][row], up to 80 KB: 2 blocks per CU) and
+ * each wave then streams column tiles of 16 on its own, with no barrier:
+ * tile b * 8 + w, then every gridDim.x * 8 further.  MFMA step s = 2t + u
+ * (v_mfma_f64_16x16x4f64) takes k = 8t + 2q + u in k-slot q, so lane j +
+ * 16q reads its column's doubles 8t + 2q, 8t + 2q + 1 with one 16-byte
+ * load (the 4 lanes of a column read 64 contiguous bytes per group of 8
+ * k), and the A operand from LDS the same k; the next group's loads are in
+ * flight during the current group's MFMAs.  k_gemm_cols runs into its
+ * K-chunk loop's latency on these shapes (256 blocks of 10 chunks at 32768
+ * columns).  The sum order over k differs from k_gemm_cols': rounding-level
+ * differences. */
+constexpr int GEMM_SK = 160;
+template <int RT>
+__global__ __launch_bounds__(512) void k_gemm_stream(int M, int K, const double* __restrict__ A, int lda,
+                                                     int ncols, const int* __restrict__ in_off,
+                                                     const int* __restrict__ out_off,
+                                                     const int* __restrict__ sexp,
+                                                     const double* __restrict__ in,
+                                                     double* __restrict__ out, int accumulate) {
+  constexpr int ROWS = 16 * RT;
+  constexpr int U = 4; /* groups of 8 k (16-byte loads per lane) in flight */
+  __shared__ double As[GEMM_SK * ROWS];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  for (int e = tid; e < K * ROWS; e += 512) {
+    const int k = e / ROWS, i = e - k * ROWS;
+    As[e] = A[(size_t)i + (size_t)k * lda];
+  }
+  __syncthreads();
+  const int j = lane & 15, q = lane >> 4;
+  const int ntiles = (ncols + 15) >> 4, kt = K >> 3;
+  for (int tile = blockIdx.x * 8 + wave; tile < ntiles; tile += gridDim.x * 8) {
+    const int col = tile * 16 + j;
+    const bool live = col < ncols;
+    const double* __restrict__ b = in + (live ? (size_t)in_off[col] : 0) + 2 * q;
+    v4d acc[RT];
+#pragma unroll
+    for (int r = 0; r < RT; r++) acc[r] = (v4d){0.0, 0.0, 0.0, 0.0};
+    double2 cur[U], nxt[U];
+    auto fetch = [&](double2 (&v)[U], int t0) {
+#pragma unroll
+      for (int u = 0; u < U; u++)
+        v[u] = (live && t0 + u < kt) ? *reinterpret_cast<const double2*>(b + 8 * (t0 + u)) : make_double2(0.0, 0.0);
+    };
+    fetch(cur, 0);
+    for (int t0 = 0; t0 < kt; t0 += U) {
+      if (t0 + U < kt) fetch(nxt, t0 + U);
+#pragma unroll
+      for (int u = 0; u < U; u++) {
+        if (t0 + u < kt) { /* wave-uniform */
+          const double* a0 = As + (8 * (t0 + u) + 2 * q) * ROWS + j;
+#pragma unroll
+          for (int r = 0; r < RT; r++) acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[16 * r], cur[u].x, acc[r], 0, 0, 0);
+#pragma unroll
+          for (int r = 0; r < RT; r++)
+            acc[r] = __builtin_amdgcn_mfma_f64_16x16x4f64(a0[ROWS + 16 * r], cur[u].y, acc[r], 0, 0, 0);
+        }
+      }
+#pragma unroll
+      for (int u = 0; u < U; u++) cur[u] = nxt[u];
+    }
+    if (!live) continue;
+    /* D[i][j] in lane j + 16 (i % 4), element i / 4 */
+    double* const o = out + out_off[col];
+    const double sc = ldexp(1.0, sexp[col]);
+#pragma unroll
+    for (int r = 0; r < RT; r++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const int row = 16 * r + q + 4 * e;
+        if (row < M) {
+          const double v = sc * acc[r][e];
+          o[row] = accumulate ? o[row] + v : v;
+        }
+      }
+  }
+}
+
 /* Launch shape of one column-list GEMM (host side, shared by the evaluator
  * and the kernel tests): the row-tile count picks the instantiation, and K
  * is split (partials reduced in a fixed order) when there are too few
- * blocks to fill the chip. */
+ * blocks to fill the chip.  Short operators over many columns take
+ * k_gemm_stream (stream = 1): its grid is col_blocks blocks of 8 waves. */
 struct GemmShape {
-  int rt, ct, row_groups, col_blocks, nsplit, ksplit, ldw;
+  int rt, ct, row_groups, col_blocks, nsplit, ksplit, ldw, stream;
 };
 inline GemmShape gemm_shape(int ncols, int M, int K) {
   GemmShape s;
+  s.stream = 0;
+  if (M <= 64 && K <= GEMM_SK && K % 8 == 0 && ncols >= 4096) {
+    s.stream = 1;
+    s.rt = (M + 15) / 16;
+    s.ct = 1;
+    s.row_groups = 1;
+    s.col_blocks = min((ncols + 127) / 128, 512); /* at most 2 blocks per CU, the rest by striding */
+    s.nsplit = 1;
+    s.ksplit = K;
+    s.ldw = (M + 7) & ~7;
+    return s;
+  }
   const int tiles = (M + 15) / 16;
   /* short operators: two 64-column tiles per block; otherwise one, with the
    * rows split into groups of <= 10 tiles.  When the inputs are short
@@ -1824,6 +1918,19 @@ inline GemmShape gemm_shape(int ncols, int M, int K) {
 inline hipError_t launch_gemm_cols(hipStream_t st, const GemmShape& sh, int M, int K, const double* A, int lda,
                                    int ncols, const int* in_off, const int* out_off, const int* sexp,
                                    const double* in, double* out, int accumulate, double* ws) {
+  if (sh.stream) {
+#define KIFMM_GEMM_STREAM(R)                                                                                   \
+  if (sh.rt == R)                                                                                              \
+    hipLaunchKernelGGL(k_gemm_stream<R>, dim3(sh.col_blocks), dim3(512), 0, st, M, K, A, lda, ncols, in_off,   \
+                       out_off, sexp, in, out, accumulate);
+    KIFMM_GEMM_STREAM(1)
+    else KIFMM_GEMM_STREAM(2)
+    else KIFMM_GEMM_STREAM(3)
+    else KIFMM_GEMM_STREAM(4)
+    else return hipErrorInvalidValue;
+#undef KIFMM_GEMM_STREAM
+    return hipGetLastError();
+  }
   const dim3 grid(sh.col_blocks, sh.row_groups, sh.nsplit);
   double* w = sh.nsplit > 1 ? ws : nullptr;
   const int ks = sh.nsplit > 1 ? sh.ksplit : K;
