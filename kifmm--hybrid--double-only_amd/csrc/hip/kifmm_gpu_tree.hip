@@ -44,22 +44,47 @@ __device__ __forceinline__ double centre(int pathc, int dep) {
   return -1.0 + 2.0 * (pathc + 0.5) / (double)(1 << dep); /* exact: dyadic */
 }
 
-/* points whose node lies in [beg, end) and was split move to a child */
-__global__ void k_assign(int n, const double* __restrict__ x, const double* __restrict__ y,
-                         const double* __restrict__ z, int* __restrict__ node, int beg, int end,
-                         const int* __restrict__ child, const int* __restrict__ path, const int* __restrict__ depth,
-                         int* __restrict__ count) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
-  const int k = node[i];
-  if (k < beg || k >= end) return;
-  const int c = child[k];
-  if (c < 0) return;
-  const int d = depth[k];
-  const int o = ((x[i] >= centre(path[3 * k], d)) << 2) | ((y[i] >= centre(path[3 * k + 1], d)) << 1) |
-                (z[i] >= centre(path[3 * k + 2], d));
-  node[i] = c + o;
-  atomicAdd(&count[c + o], 1);
+/* points whose node lies in [beg, end) and was split move to a child, and
+ * count into it.  The children of the level are nodes c0 .. c0 + nc - 1.
+ * Up to ASSIGN_LDS of them (the top levels, where a few counters take every
+ * point: the first level's 16M increments of 8 counters took 62 ms) the
+ * block counts into an LDS histogram over its grid-stride share of the
+ * points and adds each nonzero bin with one atomic; deeper levels add per
+ * point.  Integer counts: the same either way. */
+constexpr int ASSIGN_LDS = 4096;
+__global__ __launch_bounds__(256) void k_assign(int n, const double* __restrict__ x, const double* __restrict__ y,
+                                                const double* __restrict__ z, int* __restrict__ node, int beg, int end,
+                                                const int* __restrict__ child, const int* __restrict__ path,
+                                                const int* __restrict__ depth, int* __restrict__ count, int c0, int nc) {
+  __shared__ int hist[ASSIGN_LDS];
+  const bool local = nc <= ASSIGN_LDS; /* block-uniform */
+  if (local) {
+    for (int e = threadIdx.x; e < nc; e += blockDim.x) hist[e] = 0;
+    __syncthreads();
+  }
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int k = node[i];
+    if (k < beg || k >= end) continue;
+    const int c = child[k];
+    if (c < 0) continue;
+    const int d = depth[k];
+    const int o = ((x[i] >= centre(path[3 * k], d)) << 2) | ((y[i] >= centre(path[3 * k + 1], d)) << 1) |
+                  (z[i] >= centre(path[3 * k + 2], d));
+    node[i] = c + o;
+    if (local) atomicAdd(&hist[c + o - c0], 1);
+    else atomicAdd(&count[c + o], 1);
+  }
+  if (local) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc; e += blockDim.x)
+      if (hist[e] != 0) atomicAdd(&count[c0 + e], hist[e]);
+  }
+}
+/* grid of k_assign: with the LDS histogram few enough blocks that its flush
+ * stays small against the points */
+inline int assign_blocks(int n, int nc) {
+  const int b = (int)std::max<long long>(1, ((long long)n + 255) / 256);
+  return nc <= ASSIGN_LDS ? std::min(b, nc <= 512 ? 1024 : 256) : b;
 }
 
 __global__ void k_split_flags(int beg, int end, int q, const int* __restrict__ count, int* __restrict__ flag) {
@@ -373,8 +398,8 @@ bool build(int device, kifmm_problem* p, double* seconds) {
     if (!ensure((size_t)next)) return false;
     hipLaunchKernelGGL(k_children, dim3(blocks(m, 256)), dim3(256), 0, st, beg, end, flag, rank, parent, child, depth,
                        path, count, tcount);
-    hipLaunchKernelGGL(k_assign, dim3(blocks(n, 256)), dim3(256), 0, st, n, xyz[0], xyz[1], xyz[2], node, beg, end,
-                       child, path, depth, count);
+    hipLaunchKernelGGL(k_assign, dim3(assign_blocks(n, next - end)), dim3(256), 0, st, n, xyz[0], xyz[1], xyz[2],
+                       node, beg, end, child, path, depth, count, end, next - end);
     GT_TRY(hipGetLastError());
     D.release(flag);
     D.release(rank);
@@ -384,9 +409,11 @@ bool build(int device, kifmm_problem* p, double* seconds) {
   }
   const int nn = end;
   /* targets binned into the source tree, level by level */
-  for (size_t L = 0; L + 1 < T.level.size(); L++)
-    hipLaunchKernelGGL(k_assign, dim3(blocks(n, 256)), dim3(256), 0, st, n, xyz[3], xyz[4], xyz[5], tnode,
-                       T.level[L], T.level[L + 1], child, path, depth, tcount);
+  for (size_t L = 0; L + 2 < T.level.size(); L++) {
+    const int c0 = T.level[L + 1], nc = T.level[L + 2] - T.level[L + 1]; /* the children of level L */
+    hipLaunchKernelGGL(k_assign, dim3(assign_blocks(n, nc)), dim3(256), 0, st, n, xyz[3], xyz[4], xyz[5], tnode,
+                       T.level[L], T.level[L + 1], child, path, depth, tcount, c0, nc);
+  }
   int* tag = D.alloc<int>(nn);
   int* sl = D.alloc<int>(nn);
   int* tl = D.alloc<int>(nn);
