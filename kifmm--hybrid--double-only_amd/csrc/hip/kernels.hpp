@@ -626,9 +626,10 @@ __global__ __launch_bounds__(256) void k_p2p_ulist_mixed(const int4* __restrict_
 }
 
 /* Mixed-precision U-list with the close pairs listed ahead (NP <= 4).
- * Coordinates are centred on the target leaf and scaled by 1 / (tau h)
- * with tau = MIX_CLOSE = 1/8 (a power of two: exact), so a pair is close,
- * r < tau h, exactly when its scaled distance r' is below 1.
+ * Coordinates are centred on the target leaf (between the two leaves of a
+ * sibling pair) and scaled by 1 / (tau h) with tau = MIX_CLOSE = 1/8 (a
+ * power of two: exact), so a pair is close, r < tau h, exactly when its
+ * scaled distance r' is below 1.
  *
  * Distances: one v_mfma_f32_16x16x32_bf16 per 16 x 16 tile on the Gram form
  *     r'^2 = |t|^2 + sum_k A[i][k] B[k][j]
