@@ -1870,7 +1870,12 @@ __global__ __launch_bounds__(512) void k_gemm_stream(int M, int K, const double*
  * and the kernel tests): the row-tile count picks the instantiation, and K
  * is split (partials reduced in a fixed order) when there are too few
  * blocks to fill the chip.  Short operators over many columns take
- * k_gemm_stream (stream = 1): its grid is col_blocks blocks of 8 waves. */
+ * k_gemm_stream (stream = 1): its grid is col_blocks blocks of 8 waves.
+ * The choice depends on the shape alone, so every stream layout gives the
+ * same bits.  (Beside a kernel that fills the LDS, as the U-list on the
+ * second stream of the overlap layouts does, the 80 KB blocks wait for CUs
+ * to drain, 2 - 11 ms against 70 us alone; the autotuner times the layouts
+ * with it.) */
 struct GemmShape {
   int rt, ct, row_groups, col_blocks, nsplit, ksplit, ldw, stream;
 };
