@@ -1882,7 +1882,8 @@ struct GemmShape {
 inline GemmShape gemm_shape(int ncols, int M, int K) {
   GemmShape s;
   s.stream = 0;
-  if (M <= 64 && K <= GEMM_SK && K % 8 == 0 && ncols >= 4096) {
+  /* from 8192 columns: 64 blocks and more (at 4096 columns its 32 blocks lost to the split-K column GEMM) */
+  if (M <= 64 && K <= GEMM_SK && K % 8 == 0 && ncols >= 8192) {
     s.stream = 1;
     s.rt = (M + 15) / 16;
     s.ct = 1;

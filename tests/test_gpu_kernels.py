@@ -64,12 +64,12 @@ def test_mfma_bf16_operand_layout():
 
 @pytest.mark.parametrize("M,Kd,n", [(56, 152, 37), (152, 448, 20), (448, 56, 33), (26, 104, 5), (296, 1216, 17),
                                     (152, 160, 64), (152, 448, 9000), (56, 1216, 8200), (98, 56, 300),
-                                    (1216, 152, 700), (208, 26, 40), (56, 152, 70001), (26, 56, 5000),
-                                    (64, 160, 4096), (16, 8, 4100), (56, 150, 4097)])
+                                    (1216, 152, 700), (208, 26, 40), (56, 152, 70001), (26, 56, 9000),
+                                    (64, 160, 8192), (16, 8, 8200), (56, 150, 8193), (56, 152, 4096)])
 def test_gemm_matches_numpy(M, Kd, n):
     """Small column counts take the split-K route (partials + ordered
     reduction), large ones the single-pass route, and short operators (<= 64
-    rows, K <= 160 after padding to a multiple of 8) over >= 4096 columns the
+    rows, K <= 160 after padding to a multiple of 8) over >= 8192 columns the
     streaming kernel, column counts that are not multiples of 16 included."""
     rng = np.random.default_rng(M * 1000 + Kd)
     A = rng.standard_normal((M, Kd))
