@@ -349,7 +349,8 @@ def tuned_nonuniform(K, steps, warmup, nv):
              "hadamard_valu_fb16", "hadamard_mfma_fb4", "hadamard_mfma_fb8", "hadamard_mfma_fb16",
              "layout0_u_last", "layout1_u_second_stream", "p2p_mixed_split", "layout2_first_chunk_early",
              "layout3_two_chunks_early", "up_gemm", "up_fused", "down_gemm", "down_fused",
-             "layout4_u_last_one_stream")
+             "layout4_u_last_one_stream", "layout5_first_chunk_early_one_stream",
+             "layout6_two_chunks_early_one_stream")
     out.update({"default": {"mpoints_s": rate(n, ms0), "ms_per_step": ms0},
                 "tuned": {"mpoints_s": rate(n, ms1), "ms_per_step": ms1, "choice": best, "sampled_rel_err": err,
                           "eager_phases_ms": {k: round(v, 4) for k, v in avg.items()}},

@@ -78,7 +78,7 @@ typedef struct kifmm_gpu_tuning {
                         KIFMM_GPU_OVERLAP); 2, 3 the first one or two chunks there, the
                         rest last; 4 all of them last, one after another on the main stream
                         (the first copies start sooner: pays when the copy back is long
-                        against the U-list) */
+                        against the U-list); 5, 6 as 2, 3 with the rest one after another */
   int up_form;       /* KIFMM_LEVEL_*: how each level's M2M (UE2UC stack, then UC2UE) runs */
   int down_form;     /* KIFMM_LEVEL_*: how each level's L2L + DC2DE runs */
   int reserved[2];
@@ -229,10 +229,10 @@ int kifmm_gpu_get_tuning(const kifmm_gpu* g, kifmm_gpu_tuning* t);
  * evaluations with the copy back of this handle's own input, reps timed
  * runs each after one warm-up, keep the fastest of each (minimum over runs)
  * and set it.  best (may be NULL) receives the choice; ms (may be NULL, n_ms
- * entries, up to 19) receives the times in the order STATIC, PERSISTENT,
+ * entries, up to 21) receives the times in the order STATIC, PERSISTENT,
  * STATIC64, VALU fb 4, 8, 16, MFMA fb 4, 8, 16, layout 0, layout 1, SPLIT,
  * layout 2, layout 3, up as column GEMMs, up fused, down as column GEMMs,
- * down fused, layout 4; -1 for a candidate not timed. */
+ * down fused, layouts 4, 5, 6; -1 for a candidate not timed. */
 int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* ms, int n_ms);
 
 /* 1 if a HIP device is usable from this process. */

@@ -1290,15 +1290,15 @@ int launch_u(kifmm_gpu* g, hipStream_t st, int c0, int c1, double* out, int accu
 /* Copy-out chunks whose U-list runs on the second stream from the start of
  * the evaluation (alongside the far field) rather than after it: none in
  * the default layout (tuning.overlap 0) and the sequential one (4), all of
- * them in the overlap layout (1), the first one or two (2, 3) in the split
- * layouts. */
+ * them in the overlap layout (1), the first one or two in the split
+ * layouts (2, 3 and their sequential forms 5, 6). */
 int early_chunks(const kifmm_gpu* g, unsigned mask) {
   if (!(mask & KIFMM_PHASE_U)) return 0;
   const int nch = (int)g->cuts.size() - 1;
   switch (g->overlap) {
     case 1: return nch;
-    case 2: return std::min(1, nch);
-    case 3: return std::min(2, nch);
+    case 2: case 5: return std::min(1, nch);
+    case 3: case 6: return std::min(2, nch);
     default: return 0;
   }
 }
@@ -1466,10 +1466,10 @@ int enqueue_near(kifmm_gpu* g, unsigned mask, double* dst) {
   if (e < nch) { /* the other chunks after the far field */
     const int late = dst ? nch - e : 1; /* without a copy: one launch */
     /* on the two streams in turn (one chunk's last blocks run beside the
-     * next chunk's first), or in layout 4 one after another on the main
-     * stream: the first copies then start sooner, which pays when the copy
-     * back is long against the U-list */
-    const bool pairs = late > 1 && g->overlap != 4;
+     * next chunk's first), or in layouts 4 - 6 one after another on the
+     * main stream: the first copies then start sooner, which pays when the
+     * copy back is long against the U-list */
+    const bool pairs = late > 1 && g->overlap < 4;
     if (pairs) {
       HIP_TRY(hipEventRecord(g->far_ev, g->stream));
       HIP_TRY(hipStreamWaitEvent(g->side, g->far_ev, 0));
@@ -1893,10 +1893,10 @@ int kifmm_gpu_set_tuning(kifmm_gpu* g, const kifmm_gpu_tuning* t) {
   if (t->p2p_form < KIFMM_P2P_STATIC || t->p2p_form > KIFMM_P2P_SPLIT ||
       (t->hadamard_fb != 4 && t->hadamard_fb != 8 && t->hadamard_fb != 16) ||
       (t->hadamard_form != KIFMM_HAD_VALU && t->hadamard_form != KIFMM_HAD_MFMA) ||
-      t->overlap < 0 || t->overlap > 4 || t->up_form < KIFMM_LEVEL_AUTO || t->up_form > KIFMM_LEVEL_FUSED ||
+      t->overlap < 0 || t->overlap > 6 || t->up_form < KIFMM_LEVEL_AUTO || t->up_form > KIFMM_LEVEL_FUSED ||
       t->down_form < KIFMM_LEVEL_AUTO || t->down_form > KIFMM_LEVEL_FUSED) {
     set_error("kifmm_gpu_set_tuning: p2p_form must be a KIFMM_P2P_* value, hadamard_fb 4, 8 or 16, "
-              "hadamard_form a KIFMM_HAD_* value, overlap 0 to 4 and up_form / down_form KIFMM_LEVEL_* "
+              "hadamard_form a KIFMM_HAD_* value, overlap 0 to 6 and up_form / down_form KIFMM_LEVEL_* "
               "values");
     return KIFMM_EINVAL;
   }
@@ -1948,8 +1948,8 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
     set_error("kifmm_gpu_autotune: null handle or reps < 1");
     return KIFMM_EINVAL;
   }
-  double t[19];
-  std::fill(t, t + 19, -1.0);
+  double t[21];
+  std::fill(t, t + 21, -1.0);
   kifmm_gpu_tuning cur{};
   kifmm_gpu_get_tuning(g, &cur);
   /* the fastest of reps timed runs after one warm-up: the U-list, the
@@ -2016,10 +2016,10 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
    * would otherwise capture here) */
   const int use_graph = g->use_graph;
   g->use_graph = 0;
-  /* slots of the layouts 0 .. 4 in t */
-  const int lslot[5] = {9, 10, 12, 13, 18};
+  /* slots of the layouts 0 .. 6 in t */
+  const int lslot[7] = {9, 10, 12, 13, 18, 19, 20};
   int best_l = 0, rc = KIFMM_OK;
-  for (int l = 0; l < 5 && rc == KIFMM_OK; l++) {
+  for (int l = 0; l < 7 && rc == KIFMM_OK; l++) {
     c.overlap = l;
     rc = kifmm_gpu_set_tuning(g, &c);
     kifmm_phase_times pt{};
@@ -2035,7 +2035,7 @@ int kifmm_gpu_autotune(kifmm_gpu* g, int reps, kifmm_gpu_tuning* best, double* m
   TRY(kifmm_gpu_set_tuning(g, &c));
   if (best) *best = c;
   if (ms)
-    for (int k = 0; k < std::min(n_ms, 19); k++) ms[k] = t[k];
+    for (int k = 0; k < std::min(n_ms, 21); k++) ms[k] = t[k];
   return KIFMM_OK;
 }
 

@@ -470,10 +470,10 @@ class GpuEvaluator:
         with 4, 8, 16, then whole evaluations with the copy back in stream
         layouts 0 and 1, then SPLIT (mixed path), then layouts 2 and 3, then
         up as column GEMMs and fused, down as column GEMMs and fused, then
-        layout 4; -1 where not timed."""
+        layouts 4, 5, 6; -1 where not timed."""
         t = GpuTuning()
-        ms = np.zeros(19)
-        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 19))
+        ms = np.zeros(21)
+        _check(lib.kifmm_gpu_autotune(self._h, reps, C.byref(t), _dptr(ms), 21))
         return {"p2p_form": t.p2p_form, "hadamard_fb": t.hadamard_fb, "hadamard_form": t.hadamard_form,
                 "overlap": t.overlap, "up_form": t.up_form, "down_form": t.down_form}, ms
 

@@ -35,7 +35,7 @@ def test_copy_out_is_the_device_result(prec):
     assert np.array_equal(own, dev)
     got, _ = ev.evaluate()  # numpy copy of the pinned buffer
     assert np.array_equal(got, dev)
-    for ov in (1, 2, 3, 4):  # on the second stream from the start: all, first, first two; 4: one stream
+    for ov in (1, 2, 3, 4, 5, 6):  # on the second stream from the start: all, first, first two; 4 - 6: one stream
         ev.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, ov)
         got, t = ev.evaluate()
         assert np.array_equal(got, dev), ov
@@ -56,7 +56,7 @@ def test_copy_out_is_the_device_result(prec):
             else:
                 assert np.array_equal(got, dev), (f, ov)
     gr = K.GpuEvaluator(prob, precision=prec, graph=True)
-    for k in range(5):
+    for k in range(7):
         gr.set_tuning(K.P2P_STATIC, 8, K.HAD_MFMA, k)  # a recapture per layout
         got, _ = gr.evaluate()
         assert np.array_equal(got, dev), ("graph replay", k)

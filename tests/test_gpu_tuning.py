@@ -42,7 +42,7 @@ def test_every_shape_gives_identical_potentials(n, dist, q, np_, c):
     ref_mfma = None
     for form in (K.HAD_VALU, K.HAD_MFMA):
         for k, (f, fb) in enumerate(SHAPES):
-            ov = (k + form) % 5  # every stream layout across the shapes
+            ov = (k + form) % 7  # every stream layout across the shapes
             ev.set_tuning(f, fb, form, ov)
             assert ev.tuning() == {"p2p_form": f, "hadamard_fb": fb, "hadamard_form": form, "overlap": ov,
                                    "up_form": K.LEVEL_AUTO, "down_form": K.LEVEL_AUTO}
@@ -86,7 +86,7 @@ def test_mixed_path_hadamard_blocks_and_forms():
 def test_invalid_tuning_is_rejected():
     ev = K.GpuEvaluator(K.Problem.create(5000, "uniform", 64, np_=4, seed=6))
     for f, fb, form, ov, up, dn in ((4, 8, 0, 0, 0, 0), (-1, 8, 0, 0, 0, 0), (0, 5, 0, 0, 0, 0), (0, 0, 0, 0, 0, 0),
-                                    (0, 8, 2, 0, 0, 0), (0, 8, -1, 0, 0, 0), (0, 8, 1, 5, 0, 0), (0, 8, 1, -1, 0, 0),
+                                    (0, 8, 2, 0, 0, 0), (0, 8, -1, 0, 0, 0), (0, 8, 1, 7, 0, 0), (0, 8, 1, -1, 0, 0),
                                     (0, 8, 1, 0, 3, 0), (0, 8, 1, 0, -1, 0), (0, 8, 1, 0, 0, 3), (0, 8, 1, 0, 0, -1)):
         with pytest.raises(K.KifmmError) as e:
             ev.set_tuning(f, fb, form, ov, up, dn)
@@ -108,11 +108,11 @@ def test_autotune_keeps_the_fastest(precision):
     else:
         forms = {K.P2P_STATIC: 0, K.P2P_SPLIT: 11}
         assert np.all(ms[1:3] == -1)
-    assert all(ms[k] > 0 for k in forms.values()) and np.all(ms[3:11] > 0) and np.all(ms[12:19] > 0)
+    assert all(ms[k] > 0 for k in forms.values()) and np.all(ms[3:11] > 0) and np.all(ms[12:21] > 0)
     assert best["p2p_form"] == min(forms, key=lambda f: ms[forms[f]])
     v = int(np.argmin(ms[3:9]))
     assert (best["hadamard_form"], best["hadamard_fb"]) == (v // 3, (4, 8, 16)[v % 3])
-    layouts = [ms[9], ms[10], ms[12], ms[13], ms[18]]
+    layouts = [ms[9], ms[10], ms[12], ms[13], ms[18], ms[19], ms[20]]
     assert best["overlap"] == int(np.argmin(layouts))
     assert best["up_form"] == (K.LEVEL_FUSED if ms[15] < ms[14] else K.LEVEL_GEMM)
     assert best["down_form"] == (K.LEVEL_FUSED if ms[17] < ms[16] else K.LEVEL_GEMM)
