@@ -22,7 +22,8 @@
  * is copied back into the problem.
  */
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
+#include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
 #include <cstdint>
@@ -291,10 +292,10 @@ bool grow(Dev& D, T*& p, size_t have, size_t want, hipStream_t st) {
 
 bool scan(Dev& D, const int* in, int* out, int n, hipStream_t st) {
   size_t tb = 0;
-  GT_TRY(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, in, out, n, st));
+  GT_TRY(rocprim::exclusive_scan(nullptr, tb, in, out, 0, (size_t)n, rocprim::plus<int>(), st));
   void* tmp = D.alloc<char>(tb);
   GT_NEED(tmp);
-  GT_TRY(hipcub::DeviceScan::ExclusiveSum(tmp, tb, in, out, n, st));
+  GT_TRY(rocprim::exclusive_scan(tmp, tb, in, out, 0, (size_t)n, rocprim::plus<int>(), st));
   GT_TRY(hipStreamSynchronize(st));
   D.release(tmp);
   return true;
@@ -308,10 +309,10 @@ bool sort_by_leaf(Dev& D, const int* key, int n, int nbits, int* perm, hipStream
   GT_NEED(kout);
   hipLaunchKernelGGL(k_iota, dim3(blocks(n, 256)), dim3(256), 0, st, n, iota);
   size_t tb = 0;
-  GT_TRY(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, key, kout, iota, perm, n, 0, nbits, st));
+  GT_TRY(rocprim::radix_sort_pairs(nullptr, tb, key, kout, iota, perm, n, 0u, (unsigned)nbits, st));
   void* tmp = D.alloc<char>(tb);
   GT_NEED(tmp);
-  GT_TRY(hipcub::DeviceRadixSort::SortPairs(tmp, tb, key, kout, iota, perm, n, 0, nbits, st));
+  GT_TRY(rocprim::radix_sort_pairs(tmp, tb, key, kout, iota, perm, n, 0u, (unsigned)nbits, st));
   GT_TRY(hipStreamSynchronize(st));
   D.release(tmp);
   D.release(iota);
@@ -479,10 +480,10 @@ bool build(int device, kifmm_problem* p, double* seconds) {
     unsigned long long* ks = D.alloc<unsigned long long>((size_t)m);
     GT_NEED(ks);
     size_t tbytes = 0;
-    GT_TRY(hipcub::DeviceRadixSort::SortKeys(nullptr, tbytes, keys[l], ks, (int)m, 0, 32 + nbits, st));
+    GT_TRY(rocprim::radix_sort_keys(nullptr, tbytes, keys[l], ks, (int)m, 0u, (unsigned)(32 + nbits), st));
     void* tmp = D.alloc<char>(tbytes);
     GT_NEED(tmp);
-    GT_TRY(hipcub::DeviceRadixSort::SortKeys(tmp, tbytes, keys[l], ks, (int)m, 0, 32 + nbits, st));
+    GT_TRY(rocprim::radix_sort_keys(tmp, tbytes, keys[l], ks, (int)m, 0u, (unsigned)(32 + nbits), st));
     int* keep = D.alloc<int>((size_t)m);
     int* pos = D.alloc<int>((size_t)m);
     int* per = D.alloc<int>((size_t)nn);
