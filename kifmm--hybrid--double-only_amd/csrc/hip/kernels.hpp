@@ -1316,60 +1316,76 @@ static __global__ __launch_bounds__(SCAN_T) void k_scan_out(const int* __restric
 /* ------------------------------------------------------------------ S2M */
 
 /* S2M check potentials (up_eval__gpu, fmm/cuda.cu:108-190 first half): one
- * block per source leaf, lane j of the block is UC check point j.  In leaf
- * units (centre c, radius h of the node: s' = (s - c) / h, a power-of-two
- * scale, exact) the check surface is the same table for every leaf (rows
- * x, y, z, |p|^2 of the radius-3 box), and the leaf's sources, staged
- * S2M_TILE at a time in LDS as (s', |s'|^2) and w, are read as wave-wide
- * broadcasts.  Per pair the Gram form
+ * wave per source leaf.  In leaf units (centre c, radius h of the node:
+ * s' = (s - c) / h, a power-of-two scale, exact) the check surface is the
+ * same table for every leaf (rows x, y, z, |p|^2 of the radius-3 box), and
+ * the leaf's sources, staged S2M_TILE at a time in LDS as (s', |s'|^2) and
+ * w, are read as broadcasts.  The wave is G phases of 64 / G lanes: phase g
+ * takes the sources k = g mod G, and a lane holds P check points (i,
+ * i + 64 / G, ...), so one LDS read of a source (40 bytes per lane) feeds P
+ * pairs; the phases' sums meet in a lane shuffle at the end.  (G, P) =
+ * (4, 7), (2, 5), (1, 5) for 98, 152, 296 check points (NP = 3, 4, 6):
+ * 88-95 % of the lane slots busy.  Per pair the Gram form
  *     r'^2 = (|p|^2 + |s'|^2) + (-2p) . s'        (1 add, 3 FMA)
  * then the rsq, the Newton step and the FMA: 9 VALU instructions against
  * 11 for coordinate differences.  The check points lie at least 2 leaf
  * units from the leaf's points (|p| <= 3 sqrt(3), |s'| <= sqrt(3)), so the
- * Gram r'^2 is within a few eps (|p| + |s'|)^2 / 4 relative: ~1e-15.
- * Block size is n_uc rounded up to whole waves (at most 320 for NP = 6). */
-constexpr int S2M_TILE = 512;
-static __global__ __launch_bounds__(320) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
-                                             const double4* __restrict__ geo, /* node centre xyz + radius */
-                                             const double* __restrict__ surf_uc, /* 4 x ld, radius-3 box */
-                                             int n_uc, int ld_uc_surf,
-                                             const double4* __restrict__ src4,
-                                             double* __restrict__ chk, int ld_chk) {
+ * Gram r'^2 is within a few eps (|p| + |s'|)^2 / 4 relative: ~1e-15. */
+constexpr int S2M_TILE = 128;
+template <int G, int P>
+static __global__ __launch_bounds__(64) void k_s2m(const int4* __restrict__ items, /* {node, src_beg, src_num, col} */
+                                                   const double4* __restrict__ geo, /* node centre xyz + radius */
+                                                   const double* __restrict__ surf_uc, /* 4 x ld, radius-3 box */
+                                                   int n_uc, int ld_uc_surf,
+                                                   const double4* __restrict__ src4,
+                                                   double* __restrict__ chk, int ld_chk) {
+  constexpr int L = 64 / G;
+  static_assert(S2M_TILE % G == 0, "whole phases per tile");
   __shared__ double4 sS[S2M_TILE];
   __shared__ double sW[S2M_TILE];
   const int4 it = items[blockIdx.x];
-  const int j = threadIdx.x, jj = min(j, n_uc - 1);
-  const double4 g = geo[it.x];
-  const double ih = 1.0 / g.w;
-  const double ax = -2.0 * surf_uc[jj], ay = -2.0 * surf_uc[ld_uc_surf + jj], az = -2.0 * surf_uc[2 * ld_uc_surf + jj];
-  const double pp = surf_uc[3 * ld_uc_surf + jj];
-  double acc[4] = {0.0, 0.0, 0.0, 0.0};
+  const int lane = threadIdx.x, g = lane / L, i = lane % L;
+  const double4 c = geo[it.x];
+  const double ih = 1.0 / c.w;
+  double ax[P], ay[P], az[P], pp[P], acc[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const int j = min(i + L * p, n_uc - 1);
+    ax[p] = -2.0 * surf_uc[j];
+    ay[p] = -2.0 * surf_uc[ld_uc_surf + j];
+    az[p] = -2.0 * surf_uc[2 * ld_uc_surf + j];
+    pp[p] = surf_uc[3 * ld_uc_surf + j];
+    acc[p] = 0.0;
+  }
   for (int base = 0; base < it.z; base += S2M_TILE) {
     const int n = min(S2M_TILE, it.z - base);
+    const int nG = (n + G - 1) / G * G; /* padded with weight-0 copies of the tile's last source */
     __syncthreads(); /* the previous tile's readers are done */
-    for (int k = j; k < n; k += blockDim.x) {
-      const double4 q = src4[it.y + base + k];
-      const double x = (q.x - g.x) * ih, y = (q.y - g.y) * ih, z = (q.z - g.z) * ih;
+    for (int k = lane; k < nG; k += 64) {
+      const double4 q = src4[it.y + base + min(k, n - 1)];
+      const double x = (q.x - c.x) * ih, y = (q.y - c.y) * ih, z = (q.z - c.z) * ih;
       sS[k] = make_double4(x, y, z, __builtin_fma(x, x, __builtin_fma(y, y, z * z)));
-      sW[k] = q.w;
+      sW[k] = k < n ? q.w : 0.0;
     }
     __syncthreads();
-    int k = 0;
-    for (; k + 4 <= n; k += 4) {
+#pragma unroll 2
+    for (int k = g; k < nG; k += G) {
+      const double4 q = sS[k];
+      const double w = sW[k];
 #pragma unroll
-      for (int u = 0; u < 4; u++) {
-        const double4 q = sS[k + u];
-        const double r2 = __builtin_fma(ax, q.x, __builtin_fma(ay, q.y, __builtin_fma(az, q.z, pp + q.w)));
-        acc[u] = __builtin_fma(sW[k + u], rsqrt2_fp64(r2), acc[u]);
+      for (int p = 0; p < P; p++) {
+        const double r2 = __builtin_fma(ax[p], q.x, __builtin_fma(ay[p], q.y, __builtin_fma(az[p], q.z, pp[p] + q.w)));
+        acc[p] = __builtin_fma(w, rsqrt2_fp64(r2), acc[p]);
       }
     }
-    for (; k < n; k++) {
-      const double4 q = sS[k];
-      const double r2 = __builtin_fma(ax, q.x, __builtin_fma(ay, q.y, __builtin_fma(az, q.z, pp + q.w)));
-      acc[0] = __builtin_fma(sW[k], rsqrt2_fp64(r2), acc[0]);
-    }
   }
-  if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = (HALF_OOFP_R * ih) * ((acc[0] + acc[1]) + (acc[2] + acc[3]));
+#pragma unroll
+  for (int p = 0; p < P; p++)
+    for (int m = L; m < 64; m *= 2) acc[p] += __shfl_xor(acc[p], m, 64);
+  if (g == 0)
+#pragma unroll
+    for (int p = 0; p < P; p++)
+      if (i + L * p < n_uc) chk[(size_t)it.w * ld_chk + i + L * p] = (HALF_OOFP_R * ih) * acc[p];
 }
 
 /* Mixed-precision S2M and L2T (NP <= 4): the same lane-per-target scheme
@@ -1400,51 +1416,68 @@ static __global__ __launch_bounds__(256) void k_src_local32(const int4* __restri
   }
 }
 
-/* LPB consecutive leaves per block in the mixed S2M and L2T (1 for leaves
+/* LPB consecutive leaves per wave in the mixed S2M and L2T (1 for leaves
  * of ~100 points and more; 4 for small leaves, whose per-block cost and
- * surface loads are then shared).  S2M: lane = check point; the leaf's
- * sources are staged in LDS as (s, |s|^2) and w, read as broadcasts. */
-template <int LPB>
-__global__ __launch_bounds__(320) void k_s2m_mixed(const int4* __restrict__ items, int n_items,
-                                                          const double4* __restrict__ geo,
-                                                          const float* __restrict__ surf_uc32, int n_uc,
-                                                          int ld_uc_surf, const float4* __restrict__ src32,
-                                                          double* __restrict__ chk, int ld_chk) {
-  __shared__ float4 sS[S2M_TILE];
-  __shared__ float sW[S2M_TILE];
-  const int j = threadIdx.x, jj = min(j, n_uc - 1);
-  const float ax = -2.0f * surf_uc32[jj], ay = -2.0f * surf_uc32[ld_uc_surf + jj],
-              az = -2.0f * surf_uc32[2 * ld_uc_surf + jj], pp = surf_uc32[3 * ld_uc_surf + jj];
+ * surface loads are then shared).  S2M: the FP64 kernel's wave layout (G
+ * source phases, P check points per lane) in FP32; the leaf's sources are
+ * staged in LDS as (s, |s|^2) and w, read as broadcasts, and the phases'
+ * FP32 sums meet in FP64. */
+constexpr int S2M_TILE32 = 256;
+template <int LPB, int G, int P>
+__global__ __launch_bounds__(64) void k_s2m_mixed(const int4* __restrict__ items, int n_items,
+                                                  const double4* __restrict__ geo,
+                                                  const float* __restrict__ surf_uc32, int n_uc,
+                                                  int ld_uc_surf, const float4* __restrict__ src32,
+                                                  double* __restrict__ chk, int ld_chk) {
+  constexpr int L = 64 / G;
+  static_assert(S2M_TILE32 % G == 0, "whole phases per tile");
+  __shared__ float4 sS[S2M_TILE32];
+  __shared__ float sW[S2M_TILE32];
+  const int lane = threadIdx.x, g = lane / L, i = lane % L;
+  float ax[P], ay[P], az[P], pp[P];
+#pragma unroll
+  for (int p = 0; p < P; p++) {
+    const int j = min(i + L * p, n_uc - 1);
+    ax[p] = -2.0f * surf_uc32[j];
+    ay[p] = -2.0f * surf_uc32[ld_uc_surf + j];
+    az[p] = -2.0f * surf_uc32[2 * ld_uc_surf + j];
+    pp[p] = surf_uc32[3 * ld_uc_surf + j];
+  }
   for (int r = 0; r < LPB; r++) {
     const int item = blockIdx.x * LPB + r;
-    if (item >= n_items) return; /* the same for every thread of the block */
+    if (item >= n_items) return; /* the same for every lane */
     const int4 it = items[item];
     const double ih = 1.0 / geo[it.x].w;
-    float acc0 = 0.0f, acc1 = 0.0f;
-    for (int base = 0; base < it.z; base += S2M_TILE) {
-      const int n = min(S2M_TILE, it.z - base);
+    float acc[P];
+#pragma unroll
+    for (int p = 0; p < P; p++) acc[p] = 0.0f;
+    for (int base = 0; base < it.z; base += S2M_TILE32) {
+      const int n = min(S2M_TILE32, it.z - base);
+      const int nG = (n + G - 1) / G * G; /* padded with weight-0 copies of the tile's last source */
       __syncthreads(); /* the previous tile's (or leaf's) readers are done */
-      for (int k = j; k < n; k += blockDim.x) {
-        const float4 q = src32[it.y + base + k];
+      for (int k = lane; k < nG; k += 64) {
+        const float4 q = src32[it.y + base + min(k, n - 1)];
         sS[k] = make_float4(q.x, q.y, q.z, __builtin_fmaf(q.x, q.x, __builtin_fmaf(q.y, q.y, q.z * q.z)));
-        sW[k] = q.w;
+        sW[k] = k < n ? q.w : 0.0f;
       }
       __syncthreads();
-      int k = 0;
-      for (; k + 2 <= n; k += 2) {
-        const float4 s0 = sS[k], s1 = sS[k + 1];
-        const float r0 = __builtin_fmaf(ax, s0.x, __builtin_fmaf(ay, s0.y, __builtin_fmaf(az, s0.z, pp + s0.w)));
-        const float r1 = __builtin_fmaf(ax, s1.x, __builtin_fmaf(ay, s1.y, __builtin_fmaf(az, s1.z, pp + s1.w)));
-        acc0 = __builtin_fmaf(sW[k], __builtin_amdgcn_rsqf(r0), acc0);
-        acc1 = __builtin_fmaf(sW[k + 1], __builtin_amdgcn_rsqf(r1), acc1);
-      }
-      if (k < n) {
-        const float4 s0 = sS[k];
-        const float r0 = __builtin_fmaf(ax, s0.x, __builtin_fmaf(ay, s0.y, __builtin_fmaf(az, s0.z, pp + s0.w)));
-        acc0 = __builtin_fmaf(sW[k], __builtin_amdgcn_rsqf(r0), acc0);
+#pragma unroll 2
+      for (int k = g; k < nG; k += G) {
+        const float4 q = sS[k];
+        const float w = sW[k];
+#pragma unroll
+        for (int p = 0; p < P; p++) {
+          const float r2 = __builtin_fmaf(ax[p], q.x, __builtin_fmaf(ay[p], q.y, __builtin_fmaf(az[p], q.z, pp[p] + q.w)));
+          acc[p] = __builtin_fmaf(w, __builtin_amdgcn_rsqf(r2), acc[p]);
+        }
       }
     }
-    if (j < n_uc) chk[(size_t)it.w * ld_chk + j] = OOFP_R * ih * ((double)acc0 + (double)acc1);
+#pragma unroll
+    for (int p = 0; p < P; p++) {
+      double t = (double)acc[p];
+      for (int m = L; m < 64; m *= 2) t += __shfl_xor(t, m, 64);
+      if (g == 0 && i + L * p < n_uc) chk[(size_t)it.w * ld_chk + i + L * p] = OOFP_R * ih * t;
+    }
   }
 }
 

@@ -1379,12 +1379,25 @@ int enqueue_far(kifmm_gpu* g, unsigned mask) {
     /* sources never sit on the check surface: no r = 0 mask */
     if (g->precision == KIFMM_PREC_MIXED) {
       /* the FP32 leaf-local source records (mixed_sources) */
-      hipLaunchKernelGGL(g->mix_lpb == 4 ? k_s2m_mixed<4> : k_s2m_mixed<1>, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb),
-                         dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items, g->n_s2m, g->geo, g->surf_uc32,
-                         g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
+      /* (phases, check points per lane) as in the FP64 S2M; NP <= 4 here (98 or 152 check points) */
+      if (g->n_uc > 160) {
+        set_error("mixed S2M: more than 160 check points");
+        return KIFMM_EUNSUP;
+      }
+      const bool small = g->n_uc <= 112;
+      auto s2m = g->mix_lpb == 4 ? (small ? k_s2m_mixed<4, 4, 7> : k_s2m_mixed<4, 2, 5>)
+                                 : (small ? k_s2m_mixed<1, 4, 7> : k_s2m_mixed<1, 2, 5>);
+      hipLaunchKernelGGL(s2m, dim3((g->n_s2m + g->mix_lpb - 1) / g->mix_lpb), dim3(64), 0, g->stream, g->s2m_items,
+                         g->n_s2m, g->geo, g->surf_uc32, g->n_uc, g->ld_surf, g->src32, g->chk, lduc);
     } else {
-      hipLaunchKernelGGL(k_s2m, dim3(g->n_s2m), dim3(64 * ((g->n_uc + 63) / 64)), 0, g->stream, g->s2m_items,
-                         g->geo, g->surf_uc, g->n_uc, g->ld_surf, g->src4, g->chk, lduc);
+      /* (phases, check points per lane) by the check surface size: 98, 152, 296 for NP 3, 4, 6 */
+      auto s2m = g->n_uc <= 112 ? k_s2m<4, 7> : g->n_uc <= 160 ? k_s2m<2, 5> : k_s2m<1, 5>;
+      if (g->n_uc > 320) {
+        set_error("S2M: more than 320 check points");
+        return KIFMM_EUNSUP;
+      }
+      hipLaunchKernelGGL(s2m, dim3(g->n_s2m), dim3(64), 0, g->stream, g->s2m_items, g->geo, g->surf_uc, g->n_uc,
+                         g->ld_surf, g->src4, g->chk, lduc);
     }
     LAUNCH_CHECK();
     TRY(gemm(g, g->s2m_gemm, g->n_e, lduc, g->UC2UE, g->lda_e, g->chk, g->uden, 0));
