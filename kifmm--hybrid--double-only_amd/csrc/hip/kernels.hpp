@@ -1321,7 +1321,7 @@ static __global__ __launch_bounds__(SCAN_T) void k_scan_out(const int* __restric
  * same table for every leaf (rows x, y, z, |p|^2 of the radius-3 box), and
  * the leaf's sources, staged S2M_TILE at a time in LDS as (s', |s'|^2) and
  * w, are read as broadcasts.  The wave is G phases of 64 / G lanes: phase g
- * takes the sources k = g mod G, and a lane holds P check points (i,
+ * takes the sources k with k mod G = g, and a lane holds P check points (i,
  * i + 64 / G, ...), so one LDS read of a source (40 bytes per lane) feeds P
  * pairs; the phases' sums meet in a lane shuffle at the end.  (G, P) =
  * (4, 7), (2, 5), (1, 5) for 98, 152, 296 check points (NP = 3, 4, 6):
