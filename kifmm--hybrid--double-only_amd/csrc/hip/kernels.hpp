@@ -1647,6 +1647,29 @@ static __global__ __launch_bounds__(256) void k_gemm_reduce(int M, int ncols, in
   }
 }
 
+/* acc[u] = sum over steps st < nsteps, in order, of the 16 x 16 x 4 FP64
+ * MFMA of A value lda(st, u) and B value ldb(st): the A values (operator
+ * entries from global memory) of KB steps are loaded before their MFMAs,
+ * so a chain of nsteps waits on about nsteps / KB load latencies instead
+ * of nsteps. */
+template <int KB, int TPW, typename LA, typename LB>
+__device__ __forceinline__ void mfma_chain(int nsteps, LA lda, LB ldb, v4d (&acc)[TPW]) {
+  for (int sb = 0; sb < nsteps; sb += KB) {
+    double a[KB][TPW];
+#pragma unroll
+    for (int st = 0; st < KB; st++)
+#pragma unroll
+      for (int u = 0; u < TPW; u++) a[st][u] = sb + st < nsteps ? lda(sb + st, u) : 0.0;
+#pragma unroll
+    for (int st = 0; st < KB; st++) {
+      if (sb + st >= nsteps) break;
+      const double b = ldb(sb + st);
+#pragma unroll
+      for (int u = 0; u < TPW; u++) acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[st][u], b, acc[u], 0, 0, 0);
+    }
+  }
+}
+
 /* One level of the downward pass with its two operator applies fused
  * (down_eval__gpu_non_leaves, fmm/cuda.cu:1682-1801): a block owns up to 16
  * parents at depth d - 1 (in_off[j] = their downward densities, out_off[j]
@@ -1685,20 +1708,29 @@ __global__ __launch_bounds__(256) void k_l2l_dc2de(int ne, int lde, const double
   const bool live = jl < jn;
   const size_t child_off = live ? (size_t)out_off[j0 + jl] + (size_t)o * lde : 0; /* this lane's column */
   /* a wave's row tiles share each B value and keep independent
-   * accumulator chains in flight */
+   * accumulator chains in flight (mfma_chain: operator loads 8 steps ahead) */
   constexpr int TPW = (RT + 3) / 4;
+  /* this lane's check values, loaded before the products (their latency
+   * hidden behind phase 1) */
+  double cv[TPW][4];
+#pragma unroll
+  for (int u = 0; u < TPW; u++)
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+      const int row = 16 * (wave + 4 * u) + kl + 4 * i;
+      cv[u][i] = (live && wave + 4 * u < RT && row < ne) ? dchk[child_off + row] : 0.0;
+    }
+  const bool trg = live && (tag[child_off / lde] & 2) != 0; /* KIFMM_TRGNODE */
   v4d acc[TPW];
 #pragma unroll
   for (int u = 0; u < TPW; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < lde; k0 += 4) {
-    const double b = Dp[jl][k0 + kl];
-#pragma unroll
-    for (int u = 0; u < TPW; u++) {
-      const int row_a = 16 * (wave + 4 * u) + (lane & 15);
-      const double a = row_a < ne ? DE2DC[(size_t)(k0 + kl) * lda_l2l + (size_t)o * lde + row_a] : 0.0;
-      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[u], 0, 0, 0);
-    }
-  }
+  mfma_chain<8>(
+      lde / 4,
+      [&](int st, int u) {
+        const int row_a = 16 * (wave + 4 * u) + (lane & 15);
+        return row_a < ne ? DE2DC[(size_t)(4 * st + kl) * lda_l2l + (size_t)o * lde + row_a] : 0.0;
+      },
+      [&](int st) { return Dp[jl][4 * st + kl]; }, acc);
 #pragma unroll
   for (int u = 0; u < TPW; u++) {
     const int r = wave + 4 * u;
@@ -1708,24 +1740,22 @@ __global__ __launch_bounds__(256) void k_l2l_dc2de(int ne, int lde, const double
       const int row = 16 * r + kl + 4 * i;
       double t = 0.0;
       if (live && row < ne) {
-        t = dchk[child_off + row] + s1 * acc[u][i];
+        t = cv[u][i] + s1 * acc[u][i];
         dchk[child_off + row] = t;
       }
       T[jl][row] = t;
     }
   }
   __syncthreads();
-  const bool trg = live && (tag[child_off / lde] & 2) != 0; /* KIFMM_TRGNODE */
 #pragma unroll
   for (int u = 0; u < TPW; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < lde; k0 += 4) {
-    const double b = T[jl][k0 + kl];
-#pragma unroll
-    for (int u = 0; u < TPW; u++) {
-      const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1); /* rows past RT tiles: unused */
-      acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(DC2DE[(size_t)(k0 + kl) * lda_e + row_a], b, acc[u], 0, 0, 0);
-    }
-  }
+  mfma_chain<8>(
+      lde / 4,
+      [&](int st, int u) {
+        const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1); /* rows past RT tiles: unused */
+        return DC2DE[(size_t)(4 * st + kl) * lda_e + row_a];
+      },
+      [&](int st) { return T[jl][4 * st + kl]; }, acc);
 #pragma unroll
   for (int u = 0; u < TPW; u++) {
     const int r = wave + 4 * u;
@@ -1774,15 +1804,13 @@ __global__ __launch_bounds__(256) void k_m2m(int ne, int lde, int lduc, const do
       Bs[j][k] = (j < jn && k < kn) ? uden[(size_t)in_off[j0 + j] + kc + k] : 0.0;
     }
     __syncthreads();
-    for (int k0 = 0; k0 < kn; k0 += 4) {
-      const double b = Bs[jl][k0 + kl];
-#pragma unroll
-      for (int u = 0; u < TPW1; u++) {
-        const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_uc - 1); /* rows past RT1 tiles: unused */
-        acc[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(UE2UC[(size_t)(kc + k0 + kl) * lda_uc + row_a], b, acc[u], 0,
-                                                      0, 0);
-      }
-    }
+    mfma_chain<8>(
+        kn / 4,
+        [&](int st, int u) {
+          const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_uc - 1); /* rows past RT1 tiles: unused */
+          return UE2UC[(size_t)(kc + 4 * st + kl) * lda_uc + row_a];
+        },
+        [&](int st) { return Bs[jl][4 * st + kl]; }, acc);
   }
   const double s1 = ldexp(1.0, d + 1), s2 = ldexp(1.0, -d);
 #pragma unroll
@@ -1796,14 +1824,13 @@ __global__ __launch_bounds__(256) void k_m2m(int ne, int lde, int lduc, const do
   v4d acc2[TPW2];
 #pragma unroll
   for (int u = 0; u < TPW2; u++) acc2[u] = (v4d){0.0, 0.0, 0.0, 0.0};
-  for (int k0 = 0; k0 < lduc; k0 += 4) {
-    const double b = C[jl][k0 + kl];
-#pragma unroll
-    for (int u = 0; u < TPW2; u++) {
-      const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1);
-      acc2[u] = __builtin_amdgcn_mfma_f64_16x16x4f64(UC2UE[(size_t)(k0 + kl) * lda_e + row_a], b, acc2[u], 0, 0, 0);
-    }
-  }
+  mfma_chain<8>(
+      lduc / 4,
+      [&](int st, int u) {
+        const int row_a = min(16 * (wave + 4 * u) + (lane & 15), lda_e - 1);
+        return UC2UE[(size_t)(4 * st + kl) * lda_e + row_a];
+      },
+      [&](int st) { return C[jl][4 * st + kl]; }, acc2);
   if (jl >= jn) return;
   double* const out = uden + out_off[j0 + jl];
 #pragma unroll

@@ -1115,14 +1115,14 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
 /* Per level, M2M (UE2UC stack, then UC2UE) and L2L + DC2DE run either as
  * two column GEMMs or as one fused kernel.  A fused block runs every k step
  * of both operators for its 16 columns with 4 waves: fine for the square
- * n_e x n_e operators of the downward pass at NP <= 4, but the UE2UC stack
- * (K = 8 n_e) is a long serial chain per block, and the column GEMMs (64
- * columns a block, K split over blocks on short levels) are faster for M2M
- * at every NP and for the NP = 6 downward pass.  That is the automatic
- * choice; KIFMM_LEVEL_GEMM / _FUSED force one form. */
+ * n_e x n_e operators of the downward pass, but the UE2UC stack (K = 8 n_e)
+ * is a long serial chain per block, and the column GEMMs (64 columns a
+ * block, K split over blocks on short levels) are faster for M2M on most
+ * inputs.  The automatic choice is GEMMs up, fused down;
+ * KIFMM_LEVEL_GEMM / _FUSED force one form. */
 bool fused_up(const kifmm_gpu* g) { return g->up_form == KIFMM_LEVEL_FUSED; }
 bool fused_down(const kifmm_gpu* g) {
-  return g->down_form == KIFMM_LEVEL_FUSED || (g->down_form == KIFMM_LEVEL_AUTO && g->np <= 4);
+  return g->down_form != KIFMM_LEVEL_GEMM; /* AUTO: the fused kernel at every NP */
 }
 
 int m2m(kifmm_gpu* g, const GemmList& A, const GemmList& B, int d) {
