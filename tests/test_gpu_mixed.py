@@ -24,6 +24,8 @@ CASES = [
     (32768, "uniform", 32, 3, 5, 0),
     (65536, "ellipsoid", 32, 4, 2, 0),
     (20000, "gaussmix", 32, 4, 3, 1),
+    (60000, "uniform", 400, 4, 4, 0),   # leaves of 200-400 sources: several S2M tiles per leaf
+    (40000, "gaussmix", 512, 3, 1, 0),  # NP 3 S2M (4 source phases) over several tiles
 ]
 
 
