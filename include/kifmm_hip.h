@@ -84,8 +84,9 @@ typedef struct kifmm_gpu_tuning {
   int reserved[2];
 } kifmm_gpu_tuning;
 enum {
-  KIFMM_LEVEL_AUTO = 0,  /* default: column GEMMs for M2M, the fused kernel for L2L + DC2DE
-                            (the faster forms measured on uniform and clustered inputs) */
+  KIFMM_LEVEL_AUTO = 0,  /* default: the fused kernel for L2L + DC2DE, and for M2M at NP <= 4
+                            (column GEMMs for M2M at NP = 6): the faster forms measured on
+                            uniform and clustered inputs */
   KIFMM_LEVEL_GEMM = 1,  /* two column GEMMs per level (K split over blocks when the level has
                             few columns, then a reduction launch) */
   KIFMM_LEVEL_FUSED = 2  /* one kernel per level: both operator applies of 16 columns per

@@ -1777,7 +1777,8 @@ __global__ __launch_bounds__(256) void k_l2l_dc2de(int ne, int lde, const double
  * v_mfma_f64_16x16x4f64 over the 16 parent columns, K in steps of 4 in
  * order (the column GEMM's order, so its sums are bit-identical to two
  * unsplit k_gemm_cols launches), the children's densities staged through LDS
- * 64 k at a time and chk kept in LDS between the phases.  RT1 / RT2: row
+ * 64 k at a time (the next chunk loaded during the current one) and chk kept
+ * in LDS between the phases.  RT1 / RT2: row
  * tiles of the two operators (lda_uc / 16, lda_e / 16). */
 constexpr int M2M_KC = 64;
 template <int RT1, int RT2>
@@ -1796,14 +1797,35 @@ __global__ __launch_bounds__(256) void k_m2m(int ne, int lde, int lduc, const do
   v4d acc[TPW1];
 #pragma unroll
   for (int u = 0; u < TPW1; u++) acc[u] = (v4d){0.0, 0.0, 0.0, 0.0};
+  /* the children's densities of the next K chunk are loaded into
+   * registers while the current chunk is multiplied */
+  constexpr int BPT = 16 * M2M_KC / 256;
+  double rb[BPT];
+  size_t boff[BPT];
+#pragma unroll
+  for (int m = 0; m < BPT; m++) {
+    const int j = (tid + 256 * m) / M2M_KC;
+    boff[m] = j < jn ? (size_t)in_off[j0 + j] : 0;
+  }
+  auto load_b = [&](int kc) {
+    const int kn = min(M2M_KC, K - kc);
+#pragma unroll
+    for (int m = 0; m < BPT; m++) {
+      const int e = tid + 256 * m, j = e / M2M_KC, k = e % M2M_KC;
+      rb[m] = (j < jn && k < kn) ? uden[boff[m] + kc + k] : 0.0;
+    }
+  };
+  load_b(0);
   for (int kc = 0; kc < K; kc += M2M_KC) {
     const int kn = min(M2M_KC, K - kc); /* a multiple of 4 (8 lde) */
     __syncthreads(); /* the previous chunk's readers are done */
-    for (int e = tid; e < 16 * M2M_KC; e += 256) {
-      const int j = e / M2M_KC, k = e % M2M_KC;
-      Bs[j][k] = (j < jn && k < kn) ? uden[(size_t)in_off[j0 + j] + kc + k] : 0.0;
+#pragma unroll
+    for (int m = 0; m < BPT; m++) {
+      const int e = tid + 256 * m;
+      Bs[e / M2M_KC][e % M2M_KC] = rb[m];
     }
     __syncthreads();
+    if (kc + M2M_KC < K) load_b(kc + M2M_KC);
     mfma_chain<8>(
         kn / 4,
         [&](int st, int u) {

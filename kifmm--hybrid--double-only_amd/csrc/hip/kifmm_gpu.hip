@@ -1115,12 +1115,14 @@ int gemm(kifmm_gpu* g, const GemmList& L, int M, int K, const double* A, int lda
 /* Per level, M2M (UE2UC stack, then UC2UE) and L2L + DC2DE run either as
  * two column GEMMs or as one fused kernel.  A fused block runs every k step
  * of both operators for its 16 columns with 4 waves: fine for the square
- * n_e x n_e operators of the downward pass, but the UE2UC stack (K = 8 n_e)
- * is a long serial chain per block, and the column GEMMs (64 columns a
- * block, K split over blocks on short levels) are faster for M2M on most
- * inputs.  The automatic choice is GEMMs up, fused down;
- * KIFMM_LEVEL_GEMM / _FUSED force one form. */
-bool fused_up(const kifmm_gpu* g) { return g->up_form == KIFMM_LEVEL_FUSED; }
+ * n_e x n_e operators of the downward pass and for the UE2UC stack (K =
+ * 8 n_e) up to NP = 4, but at NP = 6 (K = 1216) that serial chain per block
+ * is longer than the column GEMMs (64 columns a block, K split over blocks
+ * on short levels).  The automatic choice is fused down, fused up at
+ * NP <= 4 and GEMMs up at NP = 6; KIFMM_LEVEL_GEMM / _FUSED force one form. */
+bool fused_up(const kifmm_gpu* g) {
+  return g->up_form == KIFMM_LEVEL_FUSED || (g->up_form == KIFMM_LEVEL_AUTO && g->np <= 4);
+}
 bool fused_down(const kifmm_gpu* g) {
   return g->down_form != KIFMM_LEVEL_GEMM; /* AUTO: the fused kernel at every NP */
 }
