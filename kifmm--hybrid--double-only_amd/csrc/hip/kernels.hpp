@@ -2662,7 +2662,9 @@ __global__ __launch_bounds__(512) void k_dft_inv(const int* __restrict__ blocks,
 /* W- and X-list evaluation (wlist_eval__gpu, fmm/cuda.cu:2070-2197;
  * xlist_eval__gpu, fmm/cuda.cu:2380-2493): one wave per item, a lane per
  * target point, the point in registers for the whole item and its sum in
- * four lane-private accumulators (no cross-lane reduction).  Item = two int4:
+ * four lane-private accumulators (an item of at most 32 or 16 points runs
+ * 2 or 4 source phases, one per lane group, added by a lane shuffle at the
+ * end).  Item = two int4:
  *   a = {node t, first point, count <= 64, kind}: kind 0 the targets of
  *       leaf t (trg4 from tbeg[t]), kind 1 the DC check surface c + r SP of
  *       node t, results into dchk (X-list nodes with n_e or more targets);
@@ -2727,11 +2729,12 @@ __device__ __forceinline__ int wx_fetch(WxWalk& w, const int* __restrict__ lidx,
   return cnt;
 }
 
-/* acc[u] += 2 w / |x - p| over the staged sources [0, n4), n4 a multiple of 4 */
+/* acc[u] += 2 w / |x - p| over the staged sources j0 + u, j0 = first,
+ * first + stride, ... < n4 (n4, first and stride multiples of 4) */
 template <bool MASK>
-__device__ __forceinline__ void wx_sum_tile(const double4* tile, int n4, double tx, double ty, double tz,
-                                            double (&acc)[4]) {
-  for (int j = 0; j < n4; j += 4) {
+__device__ __forceinline__ void wx_sum_tile(const double4* tile, int n4, int first, int stride, double tx, double ty,
+                                            double tz, double (&acc)[4]) {
+  for (int j = first; j < n4; j += stride) {
 #pragma unroll
     for (int u = 0; u < 4; u++) {
       const double4 p = tile[j + u];
@@ -2756,7 +2759,12 @@ __global__ __launch_bounds__(64) void k_wx(const int4* __restrict__ items, const
   __shared__ double4 tile[WX_TILE];
   const int4 a = items[2 * blockIdx.x], b = items[2 * blockIdx.x + 1];
   const int lane = threadIdx.x;
-  const int pt = a.y + min(lane, a.z - 1);
+  /* an item of at most 32 (16) points splits the wave into 2 (4) source
+   * phases of 64 / G lanes: lane = phase * (64 / G) + point, phase ph takes
+   * the source quads 4 (ph + G i); the phases' sums meet in a lane shuffle */
+  const int lg = a.z <= 16 ? 2 : a.z <= 32 ? 1 : 0; /* log2 G, item-uniform */
+  const int tl = lane & ((64 >> lg) - 1), ph = lane >> (6 - lg);
+  const int pt = a.y + min(tl, a.z - 1);
   double tx, ty, tz;
   if (a.w) {
     const double4 g = geo[a.x];
@@ -2777,15 +2785,16 @@ __global__ __launch_bounds__(64) void k_wx(const int4* __restrict__ items, const
     for (int m = 0; m < WX_PER; m++) tile[lane + 64 * m] = pre[m];
     __syncthreads();
     const int next = wx_fetch<XL>(w, lidx, child, snum, sbeg, geo, surf_ue, ld_surf, n_e, uden, ld_e, src4, lane, pre);
-    if (a.w) wx_sum_tile<false>(tile, (cnt + 3) & ~3, tx, ty, tz, acc);
-    else wx_sum_tile<MASK>(tile, (cnt + 3) & ~3, tx, ty, tz, acc);
+    if (a.w) wx_sum_tile<false>(tile, (cnt + 3) & ~3, 4 * ph, 4 << lg, tx, ty, tz, acc);
+    else wx_sum_tile<MASK>(tile, (cnt + 3) & ~3, 4 * ph, 4 << lg, tx, ty, tz, acc);
     cnt = next;
   }
-  const double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
-  if (lane < a.z) {
-    if (b.z >= 0) part[(size_t)b.z * 64 + lane] = sum;
-    else if (a.w) dchk[(size_t)a.x * ld_e + a.y + lane] += HALF_OOFP_R * sum;
-    else pot[tbeg[a.x] + a.y + lane] += HALF_OOFP_R * sum;
+  double sum = (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  for (int m = 64 >> lg; m < 64; m <<= 1) sum += __shfl_xor(sum, m, 64);
+  if (ph == 0 && tl < a.z) {
+    if (b.z >= 0) part[(size_t)b.z * 64 + tl] = sum;
+    else if (a.w) dchk[(size_t)a.x * ld_e + a.y + tl] += HALF_OOFP_R * sum;
+    else pot[tbeg[a.x] + a.y + tl] += HALF_OOFP_R * sum;
   }
 }
 
